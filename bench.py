@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark: exact 2-NN queries/sec at 1M x 128 base on one MI355X.
+
+Workload (default): base N=1,000,000 and queries M=10,000, 128-dim float32
+holding integers in [0,255] (seeded synthetic SIFT-like data), resident in HBM
+before timing.  One step = one full call of the exact 2-NN path: pack base and
+queries into the MFMA operand layout, the fused BF16 MFMA distance + top-2
+kernel, and the slab merge — the same work the reference's cublas_2nn_sift
+does per call.  value = M * steps / elapsed.
+
+Also reported on the same line:
+  roofline     : the fused kernel, flops = 2*M*N*128 per launch over its mean
+                 duration from HIP events on the library's stream
+  cpu_baseline : the oracle's host_brute restatement (oracle/) on the box's
+                 host cores over a bounded query sample against the full base
+  index_qps    : queries/sec with the base packed once (sift_bf_index_*)
+
+    python bench.py [--steps K] [--warmup W] [--compute bf16|f16|f32|i8]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12, "i8": 5.0e15}
+DTYPE = {"bf16": "bf16", "f16": "f16", "f32": "f32", "i8": "i8"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(q, b, dev_idx, n_sample, threads):
+    import oracle  # the CPU leg only: timed baseline and checker
+    qs = q[:n_sample]
+    t0 = time.perf_counter()
+    s = oracle.host_brute(qs, b, threads=threads)
+    dt = time.perf_counter() - t0
+    ok = bool(np.array_equal(s[:, 2].astype(np.uint32), dev_idx[:n_sample, 0]) and
+              np.array_equal(s[:, 3].astype(np.uint32), dev_idx[:n_sample, 1]))
+    return n_sample / dt, dt, ok
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--compute", default="bf16", choices=list(PEAK))
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--m", type=int, default=10_000)
+    ap.add_argument("--cpu-queries", type=int, default=64)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
+              "2-NN path has no multi-GPU code (see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import matching_sift_amd as ms
+    from matching_sift_amd import datagen
+
+    compute = {"bf16": ms.COMPUTE_BF16, "f16": ms.COMPUTE_F16, "f32": ms.COMPUTE_F32,
+               "i8": ms.COMPUTE_I8}[args.compute]
+    log("generating data N=%d M=%d" % (args.n, args.m))
+    b = datagen.base(args.n)
+    q = datagen.queries(b, args.m)
+    h = ms.Handle(seed=1)
+    qd, bd = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b)
+    M, N = args.m, args.n
+    outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
+            ms.DeviceArray.empty((M, 4), np.float32), ms.DeviceArray.empty((M,), np.uint32))
+    L = ms.lib()
+
+    def step():
+        rc = L.sift_knn2_async(h.h, qd.ptr, bd.ptr, ms.FLOAT_DEVICE, M, N, compute, outs[0].ptr,
+                               outs[1].ptr, outs[2].ptr, outs[3].ptr, 0.8)
+        if rc != 0:
+            raise ms.SiftError(rc, L.sift_last_error(h.h).decode())
+
+    log("warmup %d" % args.warmup)
+    for _ in range(args.warmup):
+        step()
+    h.sync()
+    h.profile(True)
+    h.profile_reset()
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    h.sync()
+    elapsed = time.perf_counter() - t0
+    fused_ms, fused_n = h.profile_read("fused")
+    pack_ms, _ = h.profile_read("pack")
+    merge_ms, _ = h.profile_read("merge")
+    h.profile(False)
+    idx = outs[0].numpy()
+    value = M * args.steps / elapsed
+    fused_avg_s = fused_ms / max(1, fused_n) / 1e3
+    flops = 2.0 * M * N * 128
+    achieved = flops / fused_avg_s / 1e12
+    peak = PEAK[args.compute] / 1e12
+
+    # prebuilt index: base packed once, queries per step
+    ix = ms.BruteForceIndex(h, bd, compute=compute, type_=ms.FLOAT_DEVICE)
+    io = (outs[0], outs[1], None, None)
+    for _ in range(2):
+        ix.query(qd, outputs=io, type_=ms.FLOAT_DEVICE, blocking=False)
+    h.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ix.query(qd, outputs=io, type_=ms.FLOAT_DEVICE, blocking=False)
+    h.sync()
+    index_qps = M * args.steps / (time.perf_counter() - t1)
+    ix.close()
+
+    traffic = None
+    if os.path.exists(args.traffic_json):
+        try:
+            tj = json.load(open(args.traffic_json))
+            if tj.get("workload") == "%dx%d-%s" % (N, M, args.compute):
+                traffic = tj.get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: %d queries x %d base on %d threads" % (args.cpu_queries, N, threads))
+        qps_cpu, dt, ok = cpu_baseline(q, b, idx, args.cpu_queries, threads)
+        cpu = {"value": qps_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": "first %d queries x full %d base, host_brute restatement (oracle/), "
+                         "%.1f s; indices equal to GPU: %s" % (args.cpu_queries, N, dt, ok)}
+
+    out = {
+        "metric": "2-NN queries/sec at 1M x 128 base (exact, bit-exact vs host_brute)",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": DTYPE[args.compute],
+        "data": "synthetic (seeded SIFT-like integers in [0,255], float32 in HBM)",
+        "config": {"workload": "exact 2-NN brute force, base N=%d, queries M=%d, d=128, "
+                               "float32 inputs, %s MFMA, fused distance+top-2" % (N, M, args.compute),
+                   "n_base": N, "n_query": M, "dim": 128, "compute": args.compute},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "fused_top2", "avg_launch_ms": fused_avg_s * 1e3,
+                     "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+        "index_qps": index_qps,
+        "breakdown_ms_per_step": {"pack": pack_ms / args.steps, "fused": fused_ms / args.steps,
+                                  "merge": merge_ms / args.steps},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

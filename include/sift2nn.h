@@ -1,0 +1,182 @@
+/*
+ * sift2nn.h — C ABI of the MI355X (gfx950) 2-NN SIFT matching library.
+ *
+ * Two layers:
+ *   1. Reference-named entry points with the reference's argument order, so a
+ *      caller of Scoooooo/matching_SIFT relinks against this library.  The
+ *      cuBLAS handle becomes an opaque sift_handle_t; CUDA float4 becomes the
+ *      layout-identical sift_float4.  Each declaration cites the reference
+ *      declaration it replaces.
+ *   2. The native API (sift_*): explicit compute kind, device outputs, return
+ *      codes, optional asynchrony.
+ *
+ * Conventions (all entry points):
+ *   - Pointers named *_dev are device pointers; FLOAT_HOST inputs are host
+ *     memory (pinned or pageable).
+ *   - Return 0 on success, a negative SIFT_ERR_* code on failure; the message
+ *     is available from sift_last_error().  void entry points (kept void for
+ *     source compatibility) record the code in sift_last_status().
+ *   - A handle owns one HIP stream and grow-only scratch.  One handle per host
+ *     thread.  Reference-named calls are blocking; native calls block unless
+ *     named *_async.
+ *   - Distances are squared L2.  For descriptors holding integers in [0,255]
+ *     every compute kind except FP8 returns bit-exact indices and distances
+ *     (ties go to the lowest base index).
+ */
+#ifndef SIFT2NN_H
+#define SIFT2NN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sift_context* sift_handle_t;
+
+/* Layout-identical to CUDA/HIP float4 as used by the reference's sorted[]
+ * output: x = d1, y = d2, z = (float)idx1, w = (float)idx2. */
+typedef struct {
+    float x, y, z, w;
+} sift_float4;
+
+/* Input memory/type codes.  0..2 match the reference's type_mem enum
+ * (src/knn_brute.cu:15-19, src/lsh_h.cu:710-714); 3 and 4 are additions. */
+enum {
+    SIFT_FLOAT_HOST = 0,   /* float[n][128] in host memory                   */
+    SIFT_FLOAT_DEVICE = 1, /* float[n][128] in device memory                 */
+    SIFT_HALF_DEVICE = 2,  /* IEEE fp16 [n][128] in device memory            */
+    SIFT_BF16_DEVICE = 3,  /* bfloat16 [n][128] in device memory             */
+    SIFT_U8_DEVICE = 4     /* uint8 [n][128] in device memory (raw SIFT)     */
+};
+
+/* Arithmetic used by the fused distance + top-2 kernel. */
+enum {
+    SIFT_COMPUTE_AUTO = 0, /* integer data in [0,255] -> BF16, else FP32     */
+    SIFT_COMPUTE_BF16 = 1, /* BF16 MFMA, FP32 accumulate                     */
+    SIFT_COMPUTE_F16 = 2,  /* FP16 MFMA, FP32 accumulate                     */
+    SIFT_COMPUTE_F32 = 3,  /* FP32 MFMA                                      */
+    SIFT_COMPUTE_I8 = 4,   /* int8 MFMA on (x-128), int32 accumulate         */
+    SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank (lsh/fp8 API) */
+};
+
+/* Hash kinds for the LSH index (lsh_type in the reference, src/lsh_h.cu:716-718). */
+enum {
+    SIFT_LSH_HYPERPLANE = 0, /* random hyperplanes (reference DOT_INT_BUCKET) */
+    SIFT_LSH_LEARNED = 1     /* learned 128->256->b MLP hash                  */
+};
+
+enum {
+    SIFT_OK = 0,
+    SIFT_ERR_INVALID = -1,     /* bad argument                               */
+    SIFT_ERR_OOM = -2,         /* device allocation failed                   */
+    SIFT_ERR_HIP = -3,         /* HIP runtime error                          */
+    SIFT_ERR_UNSUPPORTED = -4, /* valid but unsupported combination          */
+    SIFT_ERR_NOT_EXACT = -5    /* data outside the exact domain of the kind  */
+};
+
+/* ---------------------------------------------------------------- handle -- */
+
+/* Replaces cublasCreate/cublasDestroy of the reference's caller (src/test.cu:51-52). */
+int sift_create(sift_handle_t* handle);
+int sift_destroy(sift_handle_t handle);
+const char* sift_last_error(sift_handle_t handle); /* NULL handle: thread-local */
+int sift_last_status(void);                        /* status of the last call on this thread */
+int sift_synchronize(sift_handle_t handle);
+void* sift_get_stream(sift_handle_t handle);       /* hipStream_t */
+int sift_device_name(char* buf, int len);
+
+/* Device-memory plumbing for callers without their own HIP binding. */
+int sift_malloc(void** ptr, size_t bytes);
+int sift_free(void* ptr);
+int sift_host_alloc(void** ptr, size_t bytes); /* pinned, device-mapped */
+int sift_host_free(void* ptr);
+int sift_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int sift_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int sift_memset(void* dst_dev, int value, size_t bytes);
+
+/* Per-kernel timing with HIP events on the handle's stream (0 = off). */
+int sift_profile_enable(sift_handle_t handle, int on);
+int sift_profile_read(sift_handle_t handle, const char* kernel, double* total_ms, long* launches);
+int sift_profile_reset(sift_handle_t handle);
+
+/* ------------------------------------------------ reference entry points -- */
+
+/* Replaces int cublas_2nn_sift(...) (include/knn_brute.h:22, src/knn_brute.cu:24).
+ * Exact 2-NN of q against r, then the ratio test sqrt(d1)/sqrt(d2) < threshold:
+ * matches[i] = idx1 or UINT32_MAX.  type: SIFT_FLOAT_HOST/DEVICE/HALF/BF16/U8.
+ * matches: device pointer (or host memory for SIFT_FLOAT_HOST).
+ * stream_n, use and warps are accepted for source compatibility; the fused
+ * kernel needs no distance buffer, so they do not change the result. */
+int cublas_2nn_sift(void* q_points, void* r_points, int type, uint32_t q_n, uint32_t r_n,
+                    uint32_t* matches, float threshold, sift_handle_t handle, int stream_n,
+                    size_t use, int warps);
+
+/* Replaces int cublas_2nn_sift_batch(...) (include/knn_brute.h:23, src/knn_brute.cu:213).
+ * q_points typed by `type`; R is fp16 [r_n][128] on the device.  Q and dist are
+ * caller scratch in the reference; here they are unused and may be NULL.
+ * stream (hipStream_t or NULL) is the stream to run on. */
+int cublas_2nn_sift_batch(void* q_points, void* Q, int type, void* R, uint32_t q_n, uint32_t r_n,
+                          void* dist, uint32_t* matches, float threshold, sift_handle_t handle,
+                          void* stream, int warps);
+
+/* Replaces void device_brute(...) (include/knn_brute.h:14, src/knn_brute.cu:619).
+ * Device float inputs, device float4 output {d1, d2, idx1, idx2}; no handle
+ * (a per-device default handle is used).  Result: sift_last_status(). */
+void device_brute(const float* q_points, const float* r_points, int q_n, int r_n,
+                  sift_float4* sorted);
+
+/* Replaces void cublas_2nn_f(...) (include/knn_brute.h:18, src/knn_brute.cu:523). */
+void cublas_2nn_f(const float* q_points, const float* r_points, int q_n, int r_n,
+                  sift_float4* sorted, sift_handle_t handle);
+
+/* Replaces void cublas_2nn_brute_f(...) (include/knn_brute.h:17, src/knn_brute.cu:567).
+ * dist is the reference's caller-owned distance buffer; unused, may be NULL. */
+void cublas_2nn_brute_f(const float* q_points, const float* r_points, int q_n, int r_n,
+                        sift_float4* sorted, float* dist, sift_handle_t handle);
+
+/* Replaces int lsh_gpu(...) (include/lsh_h.h:69, src/lsh_h.cu:722): build an
+ * LSH index over r with l tables of nbits bits, query it with q, write
+ * matches[i] = idx1 of the re-ranked candidates when the ratio test passes,
+ * UINT32_MAX otherwise (also when a query has no candidate).  lsh_type selects
+ * SIFT_LSH_HYPERPLANE (0); stream_n is accepted and unused.  Hyperplanes come
+ * from the handle's seed (sift_set_seed). */
+int lsh_gpu(void* q_points, void* r_points, int type, uint32_t q_n, uint32_t r_n,
+            uint32_t* matches, float threshold, sift_handle_t handle, int stream_n, int l,
+            int lsh_type, int nbits);
+
+/* ------------------------------------------------------------ native API -- */
+
+int sift_set_seed(sift_handle_t handle, uint64_t seed);
+
+/* Exact 2-NN.  q, r typed by `type` (device pointers unless SIFT_FLOAT_HOST).
+ * Outputs (device pointers, each may be NULL):
+ *   idx  uint32[q_n][2]  nearest, second nearest (UINT32_MAX when r_n < 2)
+ *   dist float[q_n][2]   squared L2 (FLT_MAX when absent)
+ *   sorted sift_float4[q_n]  reference float4 layout
+ *   matches uint32[q_n]  idx1 if sqrt(d1)/sqrt(d2) < ratio else UINT32_MAX
+ * compute: SIFT_COMPUTE_*.  BF16/F16/I8 require integer data in [0,255]
+ * (checked; SIFT_ERR_NOT_EXACT otherwise). */
+int sift_knn2(sift_handle_t handle, const void* q, const void* r, int type, uint32_t q_n,
+              uint32_t r_n, int compute, uint32_t* idx, float* dist, sift_float4* sorted,
+              uint32_t* matches, float ratio);
+int sift_knn2_async(sift_handle_t handle, const void* q, const void* r, int type, uint32_t q_n,
+                    uint32_t r_n, int compute, uint32_t* idx, float* dist, sift_float4* sorted,
+                    uint32_t* matches, float ratio);
+
+/* Brute-force index: the base packed once in the MFMA operand layout, queried
+ * many times (removes the per-call base conversion of the reference). */
+typedef struct sift_bf_index* sift_bf_index_t;
+int sift_bf_index_build(sift_handle_t handle, const void* r, int type, uint32_t r_n, int compute,
+                        sift_bf_index_t* index);
+int sift_bf_index_query(sift_handle_t handle, sift_bf_index_t index, const void* q, int type,
+                        uint32_t q_n, uint32_t* idx, float* dist, sift_float4* sorted,
+                        uint32_t* matches, float ratio, int blocking);
+int sift_bf_index_destroy(sift_bf_index_t index);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SIFT2NN_H */

@@ -1,0 +1,192 @@
+"""Python mirror of the reference's 2-NN entry points over the C ABI.
+
+Reference: src/knn_brute.cu (cublas_2nn_sift :24, device_brute :619,
+cublas_2nn_f :523) and src/lsh_h.cu (lsh_gpu :722).  Argument meaning and
+order follow the reference; the cuBLAS handle becomes a Handle.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+_TYPE_OF_DTYPE = {np.dtype(np.float32): _lib.FLOAT_DEVICE, np.dtype(np.float16): _lib.HALF_DEVICE,
+                  np.dtype(np.uint8): _lib.U8_DEVICE}
+
+
+class DeviceArray:
+    """A device allocation made through the library (hipMalloc)."""
+
+    def __init__(self, nbytes, shape=None, dtype=None):
+        self.nbytes = int(nbytes)
+        self.shape = shape
+        self.dtype = np.dtype(dtype) if dtype is not None else None
+        p = C.c_void_p()
+        check(lib().sift_malloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def empty(cls, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        return cls(n, tuple(shape), dt)
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes, arr.shape, arr.dtype)
+        if arr.nbytes:
+            check(lib().sift_memcpy_h2d(d.ptr, arr.ctypes.data, arr.nbytes))
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().sift_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def fill(self, byte):
+        check(lib().sift_memset(self.ptr, int(byte), self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().sift_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def to_device(arr):
+    return arr if isinstance(arr, DeviceArray) else DeviceArray.from_numpy(arr)
+
+
+class Handle:
+    """Owns one HIP stream and grow-only scratch (replaces cublasHandle_t)."""
+
+    def __init__(self, seed=None):
+        h = C.c_void_p()
+        check(lib().sift_create(C.byref(h)))
+        self.h = h.value
+        if seed is not None:
+            check(lib().sift_set_seed(self.h, int(seed)), self.h)
+
+    def sync(self):
+        check(lib().sift_synchronize(self.h), self.h)
+
+    def profile(self, on=True):
+        check(lib().sift_profile_enable(self.h, 1 if on else 0), self.h)
+
+    def profile_reset(self):
+        check(lib().sift_profile_reset(self.h), self.h)
+
+    def profile_read(self, kernel):
+        ms, n = C.c_double(), C.c_long()
+        check(lib().sift_profile_read(self.h, kernel.encode(), C.byref(ms), C.byref(n)), self.h)
+        return ms.value, n.value
+
+    def close(self):
+        if self.h:
+            lib().sift_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _input_type(arr, type_=None):
+    if type_ is not None:
+        return type_
+    return _TYPE_OF_DTYPE[np.dtype(arr.dtype)]
+
+
+def knn2(handle, q, r, compute=_lib.COMPUTE_AUTO, ratio=0.8, type_=None, blocking=True,
+         outputs=None):
+    """Exact 2-NN of q[M,128] against r[N,128].
+
+    q, r: numpy arrays (copied to the device) or DeviceArray.  Returns numpy
+    (idx uint32[M,2], dist float32[M,2], sorted float32[M,4], matches uint32[M]).
+    """
+    qd, rd = to_device(q), to_device(r)
+    M = qd.shape[0] if qd.shape else 0
+    N = rd.shape[0] if rd.shape else 0
+    t = _input_type(qd, type_)
+    if outputs is None:
+        outputs = (DeviceArray.empty((M, 2), np.uint32), DeviceArray.empty((M, 2), np.float32),
+                   DeviceArray.empty((M, 4), np.float32), DeviceArray.empty((M,), np.uint32))
+    idx, dist, srt, mat = outputs
+    fn = lib().sift_knn2 if blocking else lib().sift_knn2_async
+    check(fn(handle.h, qd.ptr, rd.ptr, t, M, N, compute, idx.ptr, dist.ptr, srt.ptr, mat.ptr,
+             float(ratio)), handle.h)
+    if not blocking:
+        return outputs
+    return idx.numpy(), dist.numpy(), srt.numpy(), mat.numpy()
+
+
+class BruteForceIndex:
+    """Base packed once into the MFMA operand layout, queried many times."""
+
+    def __init__(self, handle, r, compute=_lib.COMPUTE_AUTO, type_=None):
+        self.handle = handle
+        rd = to_device(r)
+        self._keep = rd
+        ix = C.c_void_p()
+        check(lib().sift_bf_index_build(handle.h, rd.ptr, _input_type(rd, type_), rd.shape[0],
+                                        compute, C.byref(ix)), handle.h)
+        self.ix = ix.value
+
+    def query(self, q, ratio=0.8, type_=None, outputs=None, blocking=True):
+        qd = to_device(q)
+        M = qd.shape[0]
+        if outputs is None:
+            outputs = (DeviceArray.empty((M, 2), np.uint32), DeviceArray.empty((M, 2), np.float32),
+                       None, None)
+        idx, dist = outputs[0], outputs[1]
+        check(lib().sift_bf_index_query(self.handle.h, self.ix, qd.ptr, _input_type(qd, type_), M,
+                                        idx.ptr if idx else None, dist.ptr if dist else None,
+                                        None, None, float(ratio), 1 if blocking else 0),
+              self.handle.h)
+        if not blocking:
+            return outputs
+        return idx.numpy(), dist.numpy()
+
+    def close(self):
+        if self.ix:
+            lib().sift_bf_index_destroy(self.ix)
+            self.ix = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------- reference-named ----
+
+def cublas_2nn_sift(q_points, r_points, type_, q_n, r_n, matches, threshold, handle,
+                    stream_n=1, use=0, warps=1):
+    """src/knn_brute.cu:24 — matches[i] = idx1 if sqrt(d1)/sqrt(d2) < threshold."""
+    return check(lib().cublas_2nn_sift(q_points.ptr, r_points.ptr, type_, q_n, r_n, matches.ptr,
+                                       float(threshold), handle.h, stream_n, use, warps), handle.h)
+
+
+def device_brute(q_points, r_points, q_n, r_n, sorted_):
+    """src/knn_brute.cu:619 — exact 2-NN into float4 {d1, d2, idx1, idx2}."""
+    lib().device_brute(q_points.ptr, r_points.ptr, q_n, r_n, sorted_.ptr)
+    rc = lib().sift_last_status()
+    return check(rc)
+
+
+def cublas_2nn_f(q_points, r_points, q_n, r_n, sorted_, handle):
+    """src/knn_brute.cu:523."""
+    lib().cublas_2nn_f(q_points.ptr, r_points.ptr, q_n, r_n, sorted_.ptr, handle.h)
+    return check(lib().sift_last_status(), handle.h)

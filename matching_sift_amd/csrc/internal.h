@@ -1,0 +1,119 @@
+// internal.h — shared declarations of the sift2nn HIP library (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/sift2nn.h"
+
+namespace sift {
+
+// ------------------------------------------------------------- errors ----
+int set_error(sift_context* h, int code, const std::string& msg);
+#define SIFT_HIP_TRY(h, expr)                                                              \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            return ::sift::set_error((h), SIFT_ERR_HIP,                                    \
+                                     std::string(#expr ": ") + hipGetErrorString(_e));     \
+    } while (0)
+#define SIFT_TRY(expr)               \
+    do {                             \
+        int _rc = (expr);            \
+        if (_rc != SIFT_OK) return _rc; \
+    } while (0)
+
+// ------------------------------------------------------ device scratch ----
+// Grow-only device buffer owned by a handle: after the first call at a given
+// size no further allocation happens.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+int reserve(sift_context* h, DevBuf& b, size_t bytes);
+void release(DevBuf& b);
+
+// ---------------------------------------------------------- profiling ----
+enum ProfKernel { PROF_PACK = 0, PROF_FUSED, PROF_MERGE, PROF_LSH_PROJ, PROF_LSH_SORT,
+                  PROF_LSH_QUERY, PROF_FP8, PROF_PAIRS, PROF_MLP, PROF_COUNT };
+const char* prof_name(int k);
+// RAII pair of events around one launch on the handle's stream.
+struct ProfScope {
+    sift_context* h;
+    int k;
+    ProfScope(sift_context* h_, int k_);
+    ~ProfScope();
+};
+void prof_flush(sift_context* h);  // accumulate completed event pairs (syncs)
+
+// Packed operand layout: a "tile" is 32 rows of 128 dims stored in MFMA
+// operand order, [group][64 lanes][16 bytes]; lane l holds row l%32.
+enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4 };
+constexpr int TILE_ROWS = 32;
+inline int groups_per_tile(int kind) {
+    return kind == KIND_F32 ? 16 : (kind == KIND_I8 ? 4 : 8);
+}
+inline size_t tile_bytes(int kind) { return (size_t)groups_per_tile(kind) * 64 * 16; }
+
+struct PackedSet {
+    DevBuf tiles;  // [tiles][groups][64][16B]
+    DevBuf norm;   // float[rows_pad] (int32 for KIND_I8), natural row order
+    DevBuf norm_lane;  // [tiles][32] norms in accumulator-lane order (base only)
+    uint32_t rows = 0, rows_pad = 0, ntiles = 0;
+    int kind = 0;
+};
+
+// Pack n rows of `type` input into `kind` tiles padded to rows_pad rows.
+// Pad rows of a base set get norm = +inf (INT32_MAX for I8) so they never rank.
+// `bad` (device int) is OR-ed with 1 if a value is not an integer in [0,255].
+int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
+              bool is_base, PackedSet& out, int* bad_dev);
+// Check that every value is an integer in [0,255]; result in *bad_dev (OR-ed).
+int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_dev);
+
+struct Knn2Out {
+    uint32_t* idx = nullptr;     // [q][2]
+    float* dist = nullptr;       // [q][2]
+    sift_float4* sorted = nullptr;
+    uint32_t* matches = nullptr;
+    float ratio = 0.8f;
+};
+// Fused distance + top-2 over packed sets, then merge into the outputs.
+int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
+                uint32_t r_n, const Knn2Out& out);
+
+// Host pointer for SIFT_FLOAT_HOST input: stage it into a device buffer.
+int stage_host_input(sift_context* h, const void* host, uint32_t n, DevBuf& buf,
+                     const void** dev);
+size_t input_row_bytes(int type);
+
+}  // namespace sift
+
+struct sift_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t seed = 0x5eed2022ull;
+    std::string last_error;
+
+    sift::PackedSet q_set, b_set;
+    sift::DevBuf part;      // fused-kernel partial top-2
+    sift::DevBuf flag;      // int flags
+    sift::DevBuf stage_q, stage_r;  // host-input staging
+
+    // profiling
+    bool profile = false;
+    double prof_ms[sift::PROF_COUNT] = {};
+    long prof_n[sift::PROF_COUNT] = {};
+    struct Pending {
+        hipEvent_t a, b;
+        int k;
+    };
+    Pending* pend = nullptr;
+    int npend = 0, cap_pend = 0;
+
+    // LSH / learned-hash state lives in their own translation units
+    void* lsh_state = nullptr;
+};

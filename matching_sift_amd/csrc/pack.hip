@@ -1,0 +1,263 @@
+// pack.hip — convert descriptors into the MFMA operand layout (HBM-bound).
+//
+// Replaces the reference's float2half (src/knn_brute.cu:315-338): one pass
+// reads the input rows once and writes
+//   tiles     : 32-row tiles in MFMA operand order, [tile][group][lane][16 B],
+//               so the fused kernel's operand loads are 1 KB contiguous per
+//               wave instruction;
+//   norm      : per-row squared norm (natural order);
+//   norm_lane : per-tile norms in the order the 32x32 accumulator holds rows
+//               (lane half h, register j -> row 8*(j/4) + 4*h + j%4).
+// Conversions are round-to-nearest-even (exact for integers in [0,255]).
+#include "internal.h"
+
+namespace sift {
+namespace {
+
+template <int TYPE>
+__device__ inline void load8(const void* in, size_t row, int blk, float v[8]) {
+    if constexpr (TYPE == SIFT_FLOAT_DEVICE || TYPE == SIFT_FLOAT_HOST) {
+        const float4* p = reinterpret_cast<const float4*>(static_cast<const float*>(in) + row * 128 + blk * 8);
+        float4 a = p[0], b = p[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else if constexpr (TYPE == SIFT_HALF_DEVICE) {
+        uint4 r = *reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(in) + row * 128 + blk * 8);
+        uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            _Float16 lo, hi;
+            uint16_t l16 = (uint16_t)(w[i] & 0xFFFF), h16 = (uint16_t)(w[i] >> 16);
+            __builtin_memcpy(&lo, &l16, 2);
+            __builtin_memcpy(&hi, &h16, 2);
+            v[2 * i] = (float)lo;
+            v[2 * i + 1] = (float)hi;
+        }
+    } else if constexpr (TYPE == SIFT_BF16_DEVICE) {
+        uint4 r = *reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(in) + row * 128 + blk * 8);
+        uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[2 * i] = __uint_as_float(w[i] << 16);
+            v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+        }
+    } else {  // SIFT_U8_DEVICE
+        uint2 r = *reinterpret_cast<const uint2*>(static_cast<const uint8_t*>(in) + row * 128 + blk * 8);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = (float)((r.x >> (8 * i)) & 0xFF);
+            v[4 + i] = (float)((r.y >> (8 * i)) & 0xFF);
+        }
+    }
+}
+
+__device__ inline uint32_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ inline uint32_t f2h(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t b;
+    __builtin_memcpy(&b, &h, 2);
+    return b;
+}
+__device__ inline bool exact_u8(float x) { return x >= 0.f && x <= 255.f && x == rintf(x); }
+// One atomic per wave: the lowest active lane with the condition set reports.
+__device__ inline void flag_wave(bool cond, int* flag) {
+    const unsigned long long m = __ballot(cond);
+    if (m && (int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicOr(flag, 1);
+}
+
+// 16 threads per row, thread b handles dims [8b, 8b+8).
+template <int TYPE, int KIND>
+__global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, uint32_t n,
+                                                   uint32_t rows_pad, uint4* __restrict__ tiles,
+                                                   float* __restrict__ norm,
+                                                   float* __restrict__ norm_lane, int is_base,
+                                                   int* __restrict__ bad) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row = gid >> 4;
+    const int blk = gid & 15;
+    if (row >= rows_pad) return;  // rows_pad is a multiple of 32; whole 16-lane groups exit
+    const bool real = row < n;
+    float v[8];
+    if (real) {
+        load8<TYPE>(in, row, blk, v);
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < 8; i++) ok &= exact_u8(v[i]);
+        flag_wave(!ok, bad);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; i++) v[i] = (KIND == KIND_I8) ? 128.f : 0.f;
+    }
+    const uint32_t t = row >> 5, i32 = row & 31;
+    constexpr int G = (KIND == KIND_F32) ? 16 : (KIND == KIND_I8 ? 4 : 8);
+    uint4* tile = tiles + (size_t)t * G * 64;
+    float nrm;
+    if constexpr (KIND == KIND_BF16 || KIND == KIND_F16) {
+        uint32_t w[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint32_t lo = (KIND == KIND_BF16) ? f2bf(v[2 * i]) : f2h(v[2 * i]);
+            uint32_t hi = (KIND == KIND_BF16) ? f2bf(v[2 * i + 1]) : f2h(v[2 * i + 1]);
+            w[i] = lo | (hi << 16);
+        }
+        const int g = blk >> 1, h = blk & 1;
+        tile[g * 64 + i32 + 32 * h] = make_uint4(w[0], w[1], w[2], w[3]);
+        // norm of the rounded values the MFMA sees (identical for integer data)
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float lo, hi;
+            if constexpr (KIND == KIND_BF16) {
+                lo = __uint_as_float(w[i] << 16);
+                hi = __uint_as_float(w[i] & 0xFFFF0000u);
+            } else {
+                _Float16 a, b;
+                uint16_t la = (uint16_t)(w[i] & 0xFFFF), hb = (uint16_t)(w[i] >> 16);
+                __builtin_memcpy(&a, &la, 2);
+                __builtin_memcpy(&b, &hb, 2);
+                lo = (float)a;
+                hi = (float)b;
+            }
+            s = fmaf(lo, lo, s);
+            s = fmaf(hi, hi, s);
+        }
+        nrm = s;
+    } else if constexpr (KIND == KIND_F32) {
+        // group g = blk holds k-steps 4g..4g+3; lane half h holds dims 8g + 2i + h
+        tile[blk * 64 + i32] = make_uint4(__float_as_uint(v[0]), __float_as_uint(v[2]),
+                                          __float_as_uint(v[4]), __float_as_uint(v[6]));
+        tile[blk * 64 + i32 + 32] = make_uint4(__float_as_uint(v[1]), __float_as_uint(v[3]),
+                                               __float_as_uint(v[5]), __float_as_uint(v[7]));
+        float s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; i++) s = fmaf(v[i], v[i], s);
+        nrm = s;
+    } else {  // KIND_I8: values shifted by -128; lane half h holds dims 32g + 16h + 0..15
+        int si = 0;
+        uint32_t w[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            int x = (int)v[i] - 128;
+            si += x * x;
+            w[i >> 2] |= ((uint32_t)(x & 0xFF)) << (8 * (i & 3));
+        }
+        const int g = blk >> 2, h = (blk >> 1) & 1, half = blk & 1;
+        uint2* dst = reinterpret_cast<uint2*>(tile + g * 64 + i32 + 32 * h) + half;
+        *dst = make_uint2(w[0], w[1]);
+        nrm = __int_as_float(si);
+    }
+    // reduce the 16 partial norms of this row (lanes 16k..16k+15)
+    if constexpr (KIND == KIND_I8) {
+        int s = __float_as_int(nrm);
+        s += __shfl_xor(s, 8, 16);
+        s += __shfl_xor(s, 4, 16);
+        s += __shfl_xor(s, 2, 16);
+        s += __shfl_xor(s, 1, 16);
+        if (!real && is_base) s = 0x7FFFFFFF;
+        nrm = __int_as_float(s);
+    } else {
+        float s = nrm;
+        s += __shfl_xor(s, 8, 16);
+        s += __shfl_xor(s, 4, 16);
+        s += __shfl_xor(s, 2, 16);
+        s += __shfl_xor(s, 1, 16);
+        if (!real && is_base) s = __builtin_huge_valf();
+        nrm = s;
+    }
+    if (blk == 0) {
+        norm[row] = nrm;
+        if (is_base) {
+            const int hl = (i32 >> 2) & 1, j = 4 * (i32 >> 3) + (i32 & 3);
+            norm_lane[(size_t)t * 32 + hl * 16 + j] = nrm;
+        }
+    }
+}
+
+template <int TYPE>
+__global__ __launch_bounds__(256) void check_kernel(const void* __restrict__ in, uint32_t n,
+                                                    int* __restrict__ bad) {
+    const size_t total = (size_t)n * 16;
+    bool ok = true;
+    for (size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x; gid < total;
+         gid += (size_t)gridDim.x * blockDim.x) {
+        float v[8];
+        load8<TYPE>(in, gid >> 4, (int)(gid & 15), v);
+#pragma unroll
+        for (int i = 0; i < 8; i++) ok &= exact_u8(v[i]);
+    }
+    flag_wave(!ok, bad);
+}
+
+template <int TYPE>
+int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad, int kind,
+                  bool is_base, PackedSet& o, int* bad) {
+    const size_t threads = (size_t)rows_pad * 16;
+    const unsigned grid = (unsigned)((threads + 255) / 256);
+    if (grid == 0) return SIFT_OK;
+    auto* tiles = static_cast<uint4*>(o.tiles.p);
+    auto* norm = static_cast<float*>(o.norm.p);
+    auto* nl = static_cast<float*>(o.norm_lane.p);
+    switch (kind) {
+        case KIND_BF16:
+            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            break;
+        case KIND_F16:
+            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            break;
+        case KIND_F32:
+            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            break;
+        case KIND_I8:
+            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            break;
+        default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+}  // namespace
+
+int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
+              bool is_base, PackedSet& o, int* bad_dev) {
+    if (rows_pad % TILE_ROWS) return set_error(h, SIFT_ERR_INVALID, "pack: rows_pad not a multiple of 32");
+    const uint32_t ntiles = rows_pad / TILE_ROWS;
+    SIFT_TRY(reserve(h, o.tiles, (size_t)ntiles * tile_bytes(kind)));
+    SIFT_TRY(reserve(h, o.norm, (size_t)rows_pad * 4));
+    if (is_base) SIFT_TRY(reserve(h, o.norm_lane, (size_t)rows_pad * 4));
+    o.rows = n;
+    o.rows_pad = rows_pad;
+    o.ntiles = ntiles;
+    o.kind = kind;
+    ProfScope ps(h, PROF_PACK);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE: return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
+        case SIFT_HALF_DEVICE: return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
+        case SIFT_BF16_DEVICE: return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
+        case SIFT_U8_DEVICE: return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
+        default: return set_error(h, SIFT_ERR_INVALID, "pack: unknown input type " + std::to_string(type));
+    }
+}
+
+int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_dev) {
+    if (n == 0) return SIFT_OK;
+    const size_t threads = (size_t)n * 16;
+    unsigned grid = (unsigned)std::min<size_t>((threads + 255) / 256, 4096);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE: check_kernel<SIFT_FLOAT_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, bad_dev); break;
+        case SIFT_HALF_DEVICE: check_kernel<SIFT_HALF_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, bad_dev); break;
+        case SIFT_BF16_DEVICE: check_kernel<SIFT_BF16_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, bad_dev); break;
+        case SIFT_U8_DEVICE: return SIFT_OK;  // always exact
+        default: return set_error(h, SIFT_ERR_INVALID, "check: unknown input type");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+}  // namespace sift

@@ -1,0 +1,60 @@
+/*
+ * oracle.h — CPU restatement of the reference's 2-NN semantics.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or the timed CPU baseline).  The product library never links or calls it.
+ */
+#ifndef SIFT_ORACLE_H
+#define SIFT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* host_brute (src/knn_brute.cu:801-835) with host_lenght (:873-885). */
+void oracle_host_brute(const float* q, const float* r, int q_n, int r_n, float* sorted4);
+/* Same, queries split over `threads` OpenMP threads; identical results. */
+void oracle_host_brute_mt(const float* q, const float* r, int q_n, int r_n, float* sorted4,
+                          int threads);
+/* host_brute with a planted mistake (used only to show the checks catch it):
+ * 1 = '<=' in the first-slot test (later index wins ties, like src/knn_brute.cu:739)
+ * 2 = second slot not shifted when the first slot is replaced
+ * 3 = distance accumulates a-b instead of (a-b)^2 on the last dimension
+ * 4 = loop starts at base row 1 (drops row 0) */
+void oracle_host_brute_mutant(const float* q, const float* r, int q_n, int r_n, float* sorted4,
+                              int mutation);
+/* host_sort (src/knn_brute.cu:839-870): top-2 of each row of a distance matrix. */
+void oracle_host_sort(const float* dist, int size, int array_size, float* sorted4);
+/* Integer restatement for descriptors holding integers: int64 squared L2,
+ * lexicographic (distance, index) top-2.  idx2/dist2 = UINT32_MAX if absent. */
+void oracle_knn2_int(const float* q, const float* r, int q_n, int r_n, uint32_t* idx2,
+                     uint32_t* dist2, int threads);
+/* Ratio test: matches[i] = idx1 if sqrtf(d1)/sqrtf(d2) < thr else UINT32_MAX
+ * (src/knn_brute.cu:430-442 without the unit-norm +2). */
+void oracle_ratio(const float* sorted4, int q_n, float thr, uint32_t* matches);
+
+/* ---- LSH (random hyperplanes) ---- */
+/* codes[i] bit b = (h_b . x_i < 0), exact integer arithmetic
+ * (src/lsh_h.cu:265-317 bit order).  H: nbits x 128 integer-valued floats. */
+void oracle_lsh_codes(const float* x, int n, const float* H, int nbits, uint32_t* codes);
+/* Stable sort of (code, index) pairs; perm[k] = index of k-th smallest. */
+void oracle_lsh_sort(const uint32_t* codes, int n, uint32_t* perm);
+/* Exact 2-NN of each query restricted to the union over tables t of the base
+ * rows whose code in table t equals the query's code.  codes_q[t*q_n+i],
+ * codes_r[t*r_n+j].  ncand[i] = size of the candidate union. */
+void oracle_lsh_query(const float* q, const float* r, int q_n, int r_n, int L,
+                      const uint32_t* codes_q, const uint32_t* codes_r, uint32_t* idx2,
+                      uint32_t* dist2, uint32_t* ncand, int threads);
+
+/* ---- learned hash MLP 128 -> H (ReLU) -> B ---- */
+/* y = W2 relu(W1 x + b1) + b2 in double precision. x: n x 128, W1: H x 128,
+ * W2: B x H.  hid: n x H (post-ReLU), out: n x B. */
+void oracle_mlp_forward(const float* x, int n, const float* W1, const float* b1, int H,
+                        const float* W2, const float* b2, int B, double* hid, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

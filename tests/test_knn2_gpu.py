@@ -1,0 +1,171 @@
+"""GPU parity of the exact 2-NN path against the oracle (host_brute semantics,
+src/knn_brute.cu:801-885): bit-exact indices and distances for integer data,
+lowest index on ties, through the C ABI."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from matching_sift_amd import (BF16_DEVICE, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8,
+                               COMPUTE_AUTO, FLOAT_DEVICE, FLOAT_HOST, HALF_DEVICE, U8_DEVICE,
+                               DeviceArray, BruteForceIndex, SiftError, knn2, datagen)
+from matching_sift_amd import api
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FMAX = np.float32(np.finfo(np.float32).max)
+KINDS = [COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def expected(q, r):
+    s = oracle.host_brute(q, r, threads=8)
+    return s, oracle.ratio(s, 0.8)
+
+
+def to_idx_dist(s):
+    s64 = s.astype(np.float64)
+    idx = np.where(s64[:, 2:4] == np.float64(FMAX), 4294967295.0, s64[:, 2:4]).astype(np.uint32)
+    return idx, s[:, 0:2]
+
+
+@pytest.mark.parametrize("compute", KINDS)
+def test_config0_golden(handle, config0, compute):
+    q, b = config0
+    say("config0 golden compute", compute)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=compute, ratio=0.8)
+    gold = np.load(os.path.join(GOLD, "c0_sorted.npy"))
+    np.testing.assert_array_equal(srt, gold)
+    gi, gd = to_idx_dist(gold)
+    np.testing.assert_array_equal(idx, gi)
+    np.testing.assert_array_equal(dist, gd)
+    np.testing.assert_array_equal(mat, np.load(os.path.join(GOLD, "c0_matches_r08.npy")))
+
+
+@pytest.mark.parametrize("compute", KINDS)
+@pytest.mark.parametrize("n,m", [(1, 5), (2, 3), (31, 1), (33, 70), (64, 257), (100, 1000),
+                                 (1000, 513), (4097, 300)])
+def test_ragged_sizes_with_ties(handle, compute, n, m):
+    say("ragged", compute, n, m)
+    rng = np.random.default_rng(n * 1000 + m)
+    r = rng.integers(0, 3, size=(n, 128)).astype(np.float32)  # many exact ties
+    if n > 4:
+        r[n // 2:n // 2 + 2] = r[0:2]  # duplicates
+    q = rng.integers(0, 3, size=(m, 128)).astype(np.float32)
+    q[: min(m, n)] = r[: min(m, n)]
+    s, mt = expected(q, r)
+    idx, dist, srt, mat = knn2(handle, q, r, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+
+
+@pytest.mark.parametrize("compute", KINDS)
+def test_extremes(handle, compute):
+    """Largest distances (0 vs 255 everywhere) and all-zero rows."""
+    say("extremes", compute)
+    r = np.zeros((40, 128), np.float32)
+    r[1::3] = 255.0
+    r[5, 7] = 1.0
+    q = np.stack([np.zeros(128), np.full(128, 255.0), np.eye(1, 128, 7)[0] * 255.0]).astype(np.float32)
+    s, mt = expected(q, r)
+    idx, dist, srt, mat = knn2(handle, q, r, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+
+
+def test_empty_inputs(handle):
+    q = np.ones((3, 128), np.float32)
+    idx, dist, srt, mat = knn2(handle, q, np.zeros((0, 128), np.float32), compute=COMPUTE_BF16)
+    assert (idx == 0xFFFFFFFF).all() and (srt == FMAX).all() and (mat == 0xFFFFFFFF).all()
+    out = knn2(handle, np.zeros((0, 128), np.float32), q, compute=COMPUTE_BF16)
+    assert out[0].shape == (0, 2)
+
+
+@pytest.mark.parametrize("type_", [FLOAT_DEVICE, HALF_DEVICE, BF16_DEVICE, U8_DEVICE])
+def test_input_types(handle, config0, type_):
+    q, b = config0
+    say("input type", type_)
+    gold = np.load(os.path.join(GOLD, "c0_sorted.npy"))
+    if type_ == HALF_DEVICE:
+        qq, bb = q.astype(np.float16), b.astype(np.float16)
+    elif type_ == BF16_DEVICE:
+        qq = (q.view(np.uint32) >> 16).astype(np.uint16)
+        bb = (b.view(np.uint32) >> 16).astype(np.uint16)
+    elif type_ == U8_DEVICE:
+        qq, bb = q.astype(np.uint8), b.astype(np.uint8)
+    else:
+        qq, bb = q, b
+    idx, dist, srt, mat = knn2(handle, qq, bb, compute=COMPUTE_AUTO, type_=type_)
+    np.testing.assert_array_equal(srt, gold)
+
+
+def test_reference_entry_points(handle, config0):
+    q, b = config0
+    gold = np.load(os.path.join(GOLD, "c0_sorted.npy"))
+    gmat = np.load(os.path.join(GOLD, "c0_matches_r08.npy"))
+    qd, bd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(b)
+    say("device_brute")
+    out = DeviceArray.empty((q.shape[0], 4), np.float32)
+    api.device_brute(qd, bd, q.shape[0], b.shape[0], out)
+    np.testing.assert_array_equal(out.numpy(), gold)
+    say("cublas_2nn_f")
+    out.fill(0)
+    api.cublas_2nn_f(qd, bd, q.shape[0], b.shape[0], out, handle)
+    np.testing.assert_array_equal(out.numpy(), gold)
+    say("cublas_2nn_sift")
+    m = DeviceArray.empty((q.shape[0],), np.uint32)
+    api.cublas_2nn_sift(qd, bd, FLOAT_DEVICE, q.shape[0], b.shape[0], m, 0.8, handle, 1,
+                        2_000_000 * 383, 17)
+    np.testing.assert_array_equal(m.numpy(), gmat)
+    say("cublas_2nn_sift half")
+    qh, bh = DeviceArray.from_numpy(q.astype(np.float16)), DeviceArray.from_numpy(b.astype(np.float16))
+    m.fill(0)
+    api.cublas_2nn_sift(qh, bh, HALF_DEVICE, q.shape[0], b.shape[0], m, 0.8, handle, 1, 0, 17)
+    np.testing.assert_array_equal(m.numpy(), gmat)
+
+
+def test_brute_force_index_reuse(handle, config0):
+    q, b = config0
+    gold = np.load(os.path.join(GOLD, "c0_sorted.npy"))
+    gi, gd = to_idx_dist(gold)
+    ix = BruteForceIndex(handle, b, compute=COMPUTE_BF16)
+    for part in (slice(0, 1000), slice(0, 37), slice(500, 1000)):
+        idx, dist = ix.query(q[part])
+        np.testing.assert_array_equal(idx, gi[part])
+        np.testing.assert_array_equal(dist, gd[part])
+    ix.close()
+
+
+def test_i8_rejects_non_integer(handle):
+    q = np.full((2, 128), 0.5, np.float32)
+    with pytest.raises(SiftError) as e:
+        knn2(handle, q, q, compute=COMPUTE_I8)
+    assert e.value.code == -5
+
+
+def test_deterministic_repeat(handle, config0):
+    q, b = config0
+    a = knn2(handle, q, b, compute=COMPUTE_BF16)
+    c = knn2(handle, q, b, compute=COMPUTE_BF16)
+    for x, y in zip(a, c):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_unit_norm_float_data_f32(handle):
+    """Reference test data (unit-norm floats, src/helper.cu:40-58): FP32 MFMA
+    ranks by the GEMM form, so near-ties may swap; distances within 1e-5
+    relative + 1e-6 absolute of host_lenght; rank agreement >= 99%."""
+    r = datagen.unit(3000, seed=5)
+    q = datagen.unit(300, seed=6)
+    s = oracle.host_brute(q, r, threads=8)
+    idx, dist, srt, mat = knn2(handle, q, r, compute=COMPUTE_F32)
+    agree = (srt[:, 2] == s[:, 2]).mean()
+    assert agree >= 0.99
+    ok = srt[:, 2] == s[:, 2]
+    np.testing.assert_allclose(srt[ok, 0], s[ok, 0], rtol=1e-5, atol=1e-6)

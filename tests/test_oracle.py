@@ -1,0 +1,213 @@
+"""Pin the CPU oracle (oracle/, restating src/knn_brute.cu:801-885 and
+src/lsh_h.cu:265-317) before it is trusted as the GPU checker.
+
+The reference ships no test vectors, so each oracle function is checked
+against something that does not share its code: an independent numpy brute
+force (full distance matrix + lexicographic sort), hand-worked cases, closed
+forms, and planted mistakes that these checks must catch.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from matching_sift_amd import datagen
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FMAX = np.float32(np.finfo(np.float32).max)
+
+
+def numpy_top2(q, r):
+    """Independent: full int64 distance matrix, lexsort by (distance, index)."""
+    qi, ri = q.astype(np.int64), r.astype(np.int64)
+    d = (qi * qi).sum(1)[:, None] + (ri * ri).sum(1)[None, :] - 2 * qi @ ri.T
+    M, N = d.shape
+    idx = np.full((M, 2), 0xFFFFFFFF, np.uint64)
+    dist = np.full((M, 2), 0xFFFFFFFF, np.uint64)
+    for i in range(M):
+        order = np.lexsort((np.arange(N), d[i]))[:2]
+        for k, j in enumerate(order):
+            idx[i, k] = j
+            dist[i, k] = d[i, j]
+    return idx.astype(np.uint32), dist.astype(np.uint32)
+
+
+def sorted_to_int(s):
+    """float4 rows -> (idx, dist) uint32 with FLT_MAX mapped to UINT32_MAX."""
+    s64 = s.astype(np.float64)
+    none = np.float64(FMAX)
+    idx = np.where(s64[:, 2:4] == none, 4294967295.0, s64[:, 2:4]).astype(np.uint32)
+    dist = np.where(s64[:, 0:2] == none, 4294967295.0, s64[:, 0:2]).astype(np.uint32)
+    return idx, dist
+
+
+def rand_int(rng, n, hi=256, dup_from=None):
+    x = rng.integers(0, hi, size=(n, 128)).astype(np.float32)
+    return x
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 100, 257])
+def test_host_brute_matches_numpy_brute_force(n):
+    rng = np.random.default_rng(n)
+    r = rand_int(rng, n, hi=4)  # small alphabet -> many exact ties
+    r[n // 2:] = r[: n - n // 2]  # duplicated rows
+    q = np.concatenate([rand_int(rng, 20, hi=4), r[:5]])
+    want_idx, want_dist = numpy_top2(q, r)
+    got_idx, got_dist = sorted_to_int(oracle.host_brute(q, r))
+    np.testing.assert_array_equal(got_idx, want_idx)
+    np.testing.assert_array_equal(got_dist, want_dist)
+    i2, d2 = oracle.knn2_int(q, r)
+    np.testing.assert_array_equal(i2, want_idx)
+    np.testing.assert_array_equal(d2, want_dist)
+
+
+def test_hand_worked_cases():
+    r = np.zeros((8, 128), np.float32)
+    r[1, 0] = 3.0            # d = 9 from the zero query
+    r[2, 1] = 3.0            # d = 9 (tie with row 1 -> row 1 wins)
+    r[3, :4] = 1.0           # d = 4
+    r[5, :4] = 1.0           # duplicate of row 3
+    r[4] = 255.0             # far
+    r[6, 0], r[6, 1] = 2.0, 2.0   # d = 8
+    r[7, 5] = 2.0            # d = 4 (ties with rows 3, 5)
+    q = np.zeros((3, 128), np.float32)   # query 0: all zeros -> rows 0 (d=0)...
+    q[1, :4] = 1.0           # query 1: equals rows 3 and 5
+    q[2] = 255.0             # query 2: equals row 4
+    s = oracle.host_brute(q, r)
+    # query 0: row 0 is the zero vector (d=0); next best d=4 at rows 3,5,7 -> row 3
+    assert s[0].tolist() == [0.0, 4.0, 0.0, 3.0]
+    # query 1: d=0 at rows 3 and 5 -> (3, 5)
+    assert s[1].tolist() == [0.0, 0.0, 3.0, 5.0]
+    # query 2 (all 255): row 4 exact (d=0); the others, worked by hand:
+    #   row 0: 128*255^2 = 8,323,200     rows 1,2: 127*255^2 + 252^2 = 8,321,679
+    #   rows 3,5: 4*254^2 + 124*255^2 = 8,321,164 (tie -> row 3)
+    #   row 6: 2*253^2 + 126*255^2 = 8,321,168   row 7: 253^2 + 127*255^2 = 8,322,184
+    assert s[2].tolist() == [0.0, 8321164.0, 4.0, 3.0]
+    idx, dist = oracle.knn2_int(q, r)
+    assert idx.tolist() == [[0, 3], [3, 5], [4, 3]]
+    assert dist.tolist() == [[0, 4], [0, 0], [0, 8321164]]
+
+
+def test_sizes_zero_and_one():
+    q = np.ones((2, 128), np.float32)
+    s = oracle.host_brute(q, np.zeros((0, 128), np.float32))
+    assert (s == FMAX).all()
+    s = oracle.host_brute(q, np.zeros((1, 128), np.float32))
+    assert s[:, 0].tolist() == [128.0, 128.0] and (s[:, 1] == FMAX).all()
+    assert s[:, 2].tolist() == [0.0, 0.0] and (s[:, 3] == FMAX).all()
+
+
+def test_invariants_on_config0(config0):
+    q, b = config0
+    s = oracle.host_brute(q, b, threads=8)
+    assert (s[:, 0] <= s[:, 1]).all()
+    assert (s[:, 2] != s[:, 3]).all()
+    i1, i2 = s[:, 2].astype(np.int64), s[:, 3].astype(np.int64)
+    rec1 = ((q.astype(np.int64) - b[i1].astype(np.int64)) ** 2).sum(1)
+    rec2 = ((q.astype(np.int64) - b[i2].astype(np.int64)) ** 2).sum(1)
+    np.testing.assert_array_equal(rec1, s[:, 0].astype(np.int64))
+    np.testing.assert_array_equal(rec2, s[:, 1].astype(np.int64))
+    # multi-threaded baseline gives identical results
+    np.testing.assert_array_equal(s, oracle.host_brute(q, b, threads=1))
+
+
+def test_host_sort_equals_host_brute():
+    rng = np.random.default_rng(5)
+    r = rand_int(rng, 50, hi=3)
+    q = rand_int(rng, 10, hi=3)
+    d = ((q[:, None, :] - r[None, :, :]) ** 2).sum(-1).astype(np.float32)
+    np.testing.assert_array_equal(oracle.host_sort(d), oracle.host_brute(q, r))
+
+
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4])
+def test_planted_mistakes_are_caught(mutation):
+    """Each plausible slip in the restatement fails the numpy cross-check."""
+    rng = np.random.default_rng(3)
+    r = rand_int(rng, 64, hi=3)
+    r[40:48] = r[0:8]
+    q = np.concatenate([rand_int(rng, 30, hi=3), r[:8]])
+    want_idx, want_dist = numpy_top2(q, r)
+    got_idx, got_dist = sorted_to_int(oracle.host_brute_mutant(q, r, mutation))
+    assert not (np.array_equal(got_idx, want_idx) and np.array_equal(got_dist, want_dist))
+
+
+def test_ratio_closed_form(config0):
+    q, b = config0
+    s = oracle.host_brute(q, b, threads=8)
+    m = oracle.ratio(s, 0.8)
+    a = np.sqrt(s[:, 0].astype(np.float32))
+    c = np.sqrt(s[:, 1].astype(np.float32))
+    want = np.where(a / c < np.float32(0.8), s[:, 2].astype(np.uint32), 0xFFFFFFFF)
+    np.testing.assert_array_equal(m, want)
+    # d2 = 0 -> 0/0 = NaN -> no match
+    z = np.array([[0.0, 0.0, 3.0, 4.0]], np.float32)
+    assert oracle.ratio(z, 0.8)[0] == 0xFFFFFFFF
+
+
+def test_golden_config0(config0):
+    q, b = config0
+    meta = json.load(open(os.path.join(GOLD, "golden.json")))["config0"]
+    assert datagen.sha256(b) == meta["sha256_base_f32"]
+    assert datagen.sha256(q) == meta["sha256_query_f32"]
+    s = oracle.host_brute(q, b, threads=8)
+    np.testing.assert_array_equal(s, np.load(os.path.join(GOLD, "c0_sorted.npy")))
+    np.testing.assert_array_equal(oracle.ratio(s, 0.8), np.load(os.path.join(GOLD, "c0_matches_r08.npy")))
+    # integer restatement agrees with the float one (closed form: exact below 2^24)
+    idx, dist = oracle.knn2_int(q, b, threads=8)
+    gi, gd = sorted_to_int(s)
+    np.testing.assert_array_equal(idx, gi)
+    np.testing.assert_array_equal(dist, gd)
+
+
+def test_lsh_codes_closed_form(config0):
+    q, b = config0
+    H = datagen.gauss_int(16 * 128, 32.0, 127.0, seed=99).reshape(16, 128)
+    codes = oracle.lsh_codes(b[:256], H)
+    proj = b[:256].astype(np.int64) @ H.astype(np.int64).T
+    want = ((proj < 0).astype(np.uint64) << np.arange(16, dtype=np.uint64)).sum(1).astype(np.uint32)
+    np.testing.assert_array_equal(codes, want)
+    np.testing.assert_array_equal(codes, np.load(os.path.join(GOLD, "lsh_codes_256x16.npy")))
+    # hyperplanes are integers in [-127, 127] (exact in bf16 and int8)
+    assert np.abs(H).max() <= 127 and (H == np.round(H)).all()
+
+
+def test_lsh_sort_is_stable_counting_order():
+    rng = np.random.default_rng(9)
+    codes = rng.integers(0, 7, size=1000).astype(np.uint32)
+    perm = oracle.lsh_sort(codes)
+    np.testing.assert_array_equal(perm, np.argsort(codes, kind="stable").astype(np.uint32))
+
+
+def test_lsh_query_matches_candidate_brute_force():
+    rng = np.random.default_rng(11)
+    r = rand_int(rng, 300, hi=8)
+    q = rand_int(rng, 40, hi=8)
+    L = 3
+    cr = rng.integers(0, 4, size=(L, 300)).astype(np.uint32)
+    cq = rng.integers(0, 4, size=(L, 40)).astype(np.uint32)
+    idx, dist, nc = oracle.lsh_query(q, r, cq, cr)
+    for i in range(40):
+        cand = np.nonzero((cr == cq[:, i:i + 1]).any(0))[0]
+        assert nc[i] == len(cand)
+        if len(cand) == 0:
+            assert idx[i].tolist() == [0xFFFFFFFF] * 2
+            continue
+        wi, wd = numpy_top2(q[i:i + 1], r[cand])
+        exp_idx = [cand[j] if j != 0xFFFFFFFF else 0xFFFFFFFF for j in wi[0]]
+        assert idx[i].tolist() == exp_idx
+        assert dist[i].tolist() == wd[0].tolist()
+
+
+def test_mlp_forward_closed_form():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(5, 128)).astype(np.float32)
+    W1 = rng.normal(size=(16, 128)).astype(np.float32)
+    b1 = rng.normal(size=16).astype(np.float32)
+    W2 = rng.normal(size=(4, 16)).astype(np.float32)
+    b2 = rng.normal(size=4).astype(np.float32)
+    hid, out = oracle.mlp_forward(x, W1, b1, W2, b2)
+    h = np.maximum(x.astype(np.float64) @ W1.T.astype(np.float64) + b1, 0)
+    np.testing.assert_allclose(hid, h, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(out, h @ W2.T.astype(np.float64) + b2, rtol=1e-12, atol=1e-12)
