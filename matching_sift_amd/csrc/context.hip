@@ -48,7 +48,7 @@ void release(DevBuf& b) {
 }
 
 const char* prof_name(int k) {
-    static const char* names[PROF_COUNT] = {"pack", "fused", "merge", "lsh_proj", "lsh_sort",
+    static const char* names[PROF_COUNT] = {"pack", "fused", "merge", "seed", "lsh_proj", "lsh_sort",
                                             "lsh_query", "fp8", "pairs", "mlp"};
     return (k >= 0 && k < PROF_COUNT) ? names[k] : "?";
 }
@@ -140,7 +140,7 @@ int sift_destroy(sift_handle_t h) {
     sift_lsh_state_free(h);
     release(h->q_set.tiles); release(h->q_set.norm); release(h->q_set.norm_lane);
     release(h->b_set.tiles); release(h->b_set.norm); release(h->b_set.norm_lane);
-    release(h->part); release(h->flag); release(h->stage_q); release(h->stage_r);
+    release(h->part); release(h->thresh); release(h->flag); release(h->stage_q); release(h->stage_r);
     for (int i = 0; i < h->cap_pend; i++) {
         hipEventDestroy(h->pend[i].a);
         hipEventDestroy(h->pend[i].b);

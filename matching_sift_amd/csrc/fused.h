@@ -1,0 +1,125 @@
+// fused.h — device types shared by the fused distance + top-2 kernel
+// (fused.hip) and the slab merge (knn2.hip).
+#pragma once
+
+#include "internal.h"
+
+namespace sift {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+template <int KIND>
+struct KT;
+template <>
+struct KT<KIND_BF16> {
+    using acc_t = f32x16;
+    static constexpr int G = 8;
+    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+};
+template <>
+struct KT<KIND_F16> {
+    using acc_t = f32x16;
+    static constexpr int G = 8;
+    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
+                                                      __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    }
+};
+template <>
+struct KT<KIND_F32> {
+    using acc_t = f32x16;
+    static constexpr int G = 16;
+    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.x), __uint_as_float(b.x), c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.y), __uint_as_float(b.y), c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.z), __uint_as_float(b.z), c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.w), __uint_as_float(b.w), c, 0, 0, 0);
+        return c;
+    }
+};
+template <>
+struct KT<KIND_I8> {
+    using acc_t = i32x16;
+    static constexpr int G = 4;
+    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
+        return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a),
+                                                     __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
+    }
+};
+
+// Rank keys are float for the float kinds and int32 for KIND_I8; both are
+// carried as 32-bit patterns.  "none" = +inf / INT32_MAX with index UINT32_MAX.
+template <int KIND>
+struct Key {
+    using T = float;
+    __device__ static float none() { return __builtin_huge_valf(); }
+};
+template <>
+struct Key<KIND_I8> {
+    using T = int;
+    __device__ static int none() { return 0x7FFFFFFF; }
+};
+
+// Smallest float strictly greater than x (x finite or +inf).
+__device__ inline float next_up(float x) {
+    if (x == __builtin_huge_valf()) return x;
+    if (x == 0.f) return __uint_as_float(1u);
+    const uint32_t u = __float_as_uint(x);
+    return __uint_as_float(x > 0.f ? u + 1u : u - 1u);
+}
+
+template <typename T>
+__device__ inline bool lex_less(T x, uint32_t ix, T y, uint32_t iy) {
+    return x < y || (x == y && ix < iy);
+}
+
+// top-2 of {a1,a2} U {b1,b2}, both lists sorted lexicographically.
+template <typename T>
+__device__ inline void merge2(T& a1, uint32_t& ia1, T& a2, uint32_t& ia2, T b1, uint32_t ib1, T b2,
+                              uint32_t ib2) {
+    const bool bf = lex_less(b1, ib1, a1, ia1);
+    const T f = bf ? b1 : a1;
+    const uint32_t fi = bf ? ib1 : ia1;
+    const T c1 = bf ? a1 : b1;
+    const uint32_t ci1 = bf ? ia1 : ib1;
+    const T c2 = bf ? b2 : a2;
+    const uint32_t ci2 = bf ? ib2 : ia2;
+    const bool sf = lex_less(c2, ci2, c1, ci1);
+    a2 = sf ? c2 : c1;
+    ia2 = sf ? ci2 : ci1;
+    a1 = f;
+    ia1 = fi;
+}
+
+struct FusedArgs {
+    const uint4* q_tiles;   // packed queries, [qblocks][G][64]
+    const uint4* b_tiles;   // packed base, [n_tiles][G][64]
+    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32]
+    uint4* part;            // [n_slabs][q_stride] {key1, idx1, key2, idx2}
+    const float* thresh;    // per-query admission bound (key bits) or null
+    uint32_t q_n, q_stride;
+    int n_tiles, tiles_per_slab, n_slabs, n_groups;
+};
+
+
+// Launch shapes: (QW query blocks of 32 per wave, WAVES per block, TPS base
+// tiles per LDS stage).  Queries are padded to a multiple of QPAD so every
+// shape's group count fits the packed query buffer.
+constexpr uint32_t QPAD = 1024;
+using FusedFn = void (*)(const FusedArgs&, unsigned, hipStream_t);
+struct FusedShape {
+    int kind, qw, waves, tps;
+    FusedFn fn;
+};
+// SIFT_SHAPE="qw,waves,tps" picks a listed shape of the kind (for tuning);
+// otherwise the kind's default (first listed).
+const FusedShape* pick_shape(int kind);
+
+}  // namespace sift

@@ -37,7 +37,7 @@ int reserve(sift_context* h, DevBuf& b, size_t bytes);
 void release(DevBuf& b);
 
 // ---------------------------------------------------------- profiling ----
-enum ProfKernel { PROF_PACK = 0, PROF_FUSED, PROF_MERGE, PROF_LSH_PROJ, PROF_LSH_SORT,
+enum ProfKernel { PROF_PACK = 0, PROF_FUSED, PROF_MERGE, PROF_SEED, PROF_LSH_PROJ, PROF_LSH_SORT,
                   PROF_LSH_QUERY, PROF_FP8, PROF_PAIRS, PROF_MLP, PROF_COUNT };
 const char* prof_name(int k);
 // RAII pair of events around one launch on the handle's stream.
@@ -100,6 +100,7 @@ struct sift_context {
 
     sift::PackedSet q_set, b_set;
     sift::DevBuf part;      // fused-kernel partial top-2
+    sift::DevBuf thresh;    // per-query admission bounds from the seeding pass
     sift::DevBuf flag;      // int flags
     sift::DevBuf stage_q, stage_r;  // host-input staging
 

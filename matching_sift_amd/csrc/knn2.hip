@@ -1,263 +1,21 @@
-// knn2.hip — exact 2-NN: fused MFMA distance + top-2 kernel, slab merge,
-// and the brute-force entry points.
+// knn2.hip — exact 2-NN host orchestration: seeding pass, fused-kernel
+// launch (fused.hip), slab merge with the output formats, and the
+// brute-force entry points (reference src/knn_brute.cu).
 //
 // The reference materialises a q x r distance matrix with cuBLAS and re-reads
 // it in a top-2 kernel (src/knn_brute.cu:244 then :265; :592 then :606;
-// :644 then :648).  Here one kernel computes, per 32x32 MFMA tile,
-//     t = ||b||^2 - 2 q.b          (rank key; ||q||^2 is constant per query)
-// and folds it into a per-lane top-2, so no distance ever reaches HBM.
-//
-// Work decomposition (one launch):
-//   block  = (query group, base slab); WAVES waves x QW blocks of 32 queries
-//   wave   : query operands resident in VGPRs for the whole slab
-//   block  : base tiles streamed HBM/L2 -> VGPR -> LDS (one copy shared by
-//            all waves), TPS tiles per stage, next stage prefetched into
-//            registers while the current one is consumed
-//   lane   : accumulator layout of v_mfma_*_32x32: lane l holds query l%32 and
-//            16 base rows (8*(j/4) + 4*(l/32) + j%4), so each lane keeps the
-//            top-2 of its own rows for its own query; lanes l and l^32 merge
-//            at the end.
-//   block -> (group, slab) is XCD-aware: consecutive blocks land on different
-//            XCDs, so all groups of one slab are placed on the same XCD and
-//            share that XCD's L2 copy of the slab.
-// Tie-break: rows reach a lane in increasing index order and the insert uses
-// strict '<', merges compare (value, index) lexicographically — the result is
-// the reference host_brute's (src/knn_brute.cu:816-831) lowest-index rule.
+// :644 then :648); here the distance never reaches HBM and the only
+// intermediate is a per-(slab, query) top-2 partial of 16 bytes.
 #include <algorithm>
 #include <cfloat>
+#include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
-#include "internal.h"
+#include "fused.h"
 
 namespace sift {
 namespace {
-
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef int i32x16 __attribute__((ext_vector_type(16)));
-typedef int i32x4 __attribute__((ext_vector_type(4)));
-
-template <int KIND>
-struct KT;
-template <>
-struct KT<KIND_BF16> {
-    using acc_t = f32x16;
-    static constexpr int G = 8;
-    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
-                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-    }
-};
-template <>
-struct KT<KIND_F16> {
-    using acc_t = f32x16;
-    static constexpr int G = 8;
-    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
-                                                      __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
-    }
-};
-template <>
-struct KT<KIND_F32> {
-    using acc_t = f32x16;
-    static constexpr int G = 16;
-    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
-        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.x), __uint_as_float(b.x), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.y), __uint_as_float(b.y), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.z), __uint_as_float(b.z), c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.w), __uint_as_float(b.w), c, 0, 0, 0);
-        return c;
-    }
-};
-template <>
-struct KT<KIND_I8> {
-    using acc_t = i32x16;
-    static constexpr int G = 4;
-    __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
-        return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a),
-                                                     __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
-    }
-};
-
-// Rank keys are float for the float kinds and int32 for KIND_I8; both are
-// carried as 32-bit patterns.  "none" = +inf / INT32_MAX with index UINT32_MAX.
-template <int KIND>
-struct Key {
-    using T = float;
-    __device__ static float none() { return __builtin_huge_valf(); }
-};
-template <>
-struct Key<KIND_I8> {
-    using T = int;
-    __device__ static int none() { return 0x7FFFFFFF; }
-};
-
-template <typename T>
-__device__ inline bool lex_less(T x, uint32_t ix, T y, uint32_t iy) {
-    return x < y || (x == y && ix < iy);
-}
-
-// top-2 of {a1,a2} U {b1,b2}, both lists sorted lexicographically.
-template <typename T>
-__device__ inline void merge2(T& a1, uint32_t& ia1, T& a2, uint32_t& ia2, T b1, uint32_t ib1, T b2,
-                              uint32_t ib2) {
-    const bool bf = lex_less(b1, ib1, a1, ia1);
-    const T f = bf ? b1 : a1;
-    const uint32_t fi = bf ? ib1 : ia1;
-    const T c1 = bf ? a1 : b1;
-    const uint32_t ci1 = bf ? ia1 : ib1;
-    const T c2 = bf ? b2 : a2;
-    const uint32_t ci2 = bf ? ib2 : ia2;
-    const bool sf = lex_less(c2, ci2, c1, ci1);
-    a2 = sf ? c2 : c1;
-    ia2 = sf ? ci2 : ci1;
-    a1 = f;
-    ia1 = fi;
-}
-
-struct FusedArgs {
-    const uint4* q_tiles;   // packed queries, [qblocks][G][64]
-    const uint4* b_tiles;   // packed base, [n_tiles][G][64]
-    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32]
-    uint4* part;            // [n_slabs][q_stride] {key1, idx1, key2, idx2}
-    uint32_t q_n, q_stride;
-    int n_tiles, tiles_per_slab, n_slabs, n_groups;
-};
-
-template <int KIND, int QW, int WAVES, int TPS>
-__global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
-    using KTk = KT<KIND>;
-    using acc_t = typename KTk::acc_t;
-    using key_t = typename Key<KIND>::T;
-    constexpr int G = KTk::G;
-    constexpr int UPT = G * 64;               // 16-byte units per tile
-    constexpr int NT = WAVES * 64;
-    constexpr int STAGE = TPS * UPT;
-    constexpr int PER_THREAD = STAGE / NT;
-    static_assert(STAGE % NT == 0, "stage must split evenly over the block");
-    static_assert(TPS * 32 <= NT, "one norm per thread per stage");
-
-    __shared__ __align__(16) uint4 s_tiles[STAGE];
-    __shared__ __align__(16) float s_nl[TPS * 32];
-
-    const int b = blockIdx.x;
-    const int xcd = b & 7, r = b >> 3;
-    const int slab = (r / a.n_groups) * 8 + xcd;
-    const int group = r % a.n_groups;
-    if (slab >= a.n_slabs) return;  // uniform per block
-    const int t_begin = slab * a.tiles_per_slab;
-    const int t_end = min(t_begin + a.tiles_per_slab, a.n_tiles);
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, h = lane >> 5;
-    const int qblk0 = (group * WAVES + wave) * QW;
-
-    uint4 qop[QW][G];
-#pragma unroll
-    for (int qb = 0; qb < QW; qb++)
-#pragma unroll
-        for (int g = 0; g < G; g++) qop[qb][g] = a.q_tiles[((size_t)(qblk0 + qb) * G + g) * 64 + lane];
-
-    key_t m1[QW], m2[QW];
-    uint32_t i1[QW], i2[QW];
-#pragma unroll
-    for (int qb = 0; qb < QW; qb++) {
-        m1[qb] = m2[qb] = Key<KIND>::none();
-        i1[qb] = i2[qb] = 0xFFFFFFFFu;
-    }
-
-    uint4 pre[PER_THREAD];
-    float pre_nl = 0.f;
-    auto load_stage = [&](int ts) {
-        const int avail = min(TPS, t_end - ts);
-        const uint4* src = a.b_tiles + (size_t)ts * UPT;
-#pragma unroll
-        for (int u = 0; u < PER_THREAD; u++) {
-            const int idx = u * NT + tid;
-            pre[u] = (idx < avail * UPT) ? src[idx] : make_uint4(0u, 0u, 0u, 0u);
-        }
-        if (tid < TPS * 32) pre_nl = (tid < avail * 32) ? a.b_nl[(size_t)ts * 32 + tid] : 0.f;
-    };
-
-    if (t_begin < t_end) load_stage(t_begin);
-    for (int ts = t_begin; ts < t_end; ts += TPS) {
-        __syncthreads();  // previous stage fully consumed
-#pragma unroll
-        for (int u = 0; u < PER_THREAD; u++) s_tiles[u * NT + tid] = pre[u];
-        if (tid < TPS * 32) s_nl[tid] = pre_nl;
-        __syncthreads();
-        if (ts + TPS < t_end) load_stage(ts + TPS);  // in flight during this stage
-        const int ntile = min(TPS, t_end - ts);
-#pragma unroll
-        for (int tt = 0; tt < TPS; tt++) {
-            if (tt >= ntile) break;
-            acc_t acc[QW];
-#pragma unroll
-            for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
-#pragma unroll
-            for (int g = 0; g < G; g++) {
-                const uint4 av = s_tiles[tt * UPT + g * 64 + lane];
-#pragma unroll
-                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(av, qop[qb][g], acc[qb]);
-            }
-            // norms of this lane's 16 rows
-            const float4* nlp = reinterpret_cast<const float4*>(s_nl + tt * 32 + h * 16);
-            float nl[16];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float4 v = nlp[c];
-                nl[4 * c] = v.x; nl[4 * c + 1] = v.y; nl[4 * c + 2] = v.z; nl[4 * c + 3] = v.w;
-            }
-            const uint32_t rowbase = (uint32_t)(ts + tt) * 32u + 4u * h;
-#pragma unroll
-            for (int qb = 0; qb < QW; qb++) {
-                key_t tv[16];
-#pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    if constexpr (KIND == KIND_I8)
-                        tv[j] = __float_as_int(nl[j]) - 2 * acc[qb][j];
-                    else
-                        tv[j] = fmaf(acc[qb][j], -2.f, nl[j]);
-                }
-                key_t mn = tv[0];
-#pragma unroll
-                for (int j = 1; j < 16; j++) mn = min(mn, tv[j]);
-                if (mn < m2[qb]) {  // rare after warm-up: some row beats the lane's 2nd best
-#pragma unroll
-                    for (int j = 0; j < 16; j++) {
-                        const key_t v = tv[j];
-                        const uint32_t row = rowbase + 8u * (j >> 2) + (j & 3);
-                        const bool c1 = v < m1[qb], c2 = v < m2[qb];
-                        m2[qb] = c1 ? m1[qb] : (c2 ? v : m2[qb]);
-                        i2[qb] = c1 ? i1[qb] : (c2 ? row : i2[qb]);
-                        m1[qb] = c1 ? v : m1[qb];
-                        i1[qb] = c1 ? row : i1[qb];
-                    }
-                }
-            }
-        }
-    }
-
-    // lanes l and l^32 hold disjoint rows of the same query
-#pragma unroll
-    for (int qb = 0; qb < QW; qb++) {
-        const key_t o1 = __shfl_xor(m1[qb], 32);
-        const key_t o2 = __shfl_xor(m2[qb], 32);
-        const uint32_t oi1 = __shfl_xor(i1[qb], 32);
-        const uint32_t oi2 = __shfl_xor(i2[qb], 32);
-        merge2(m1[qb], i1[qb], m2[qb], i2[qb], o1, oi1, o2, oi2);
-        const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + (uint32_t)lane;
-        if (h == 0 && q < a.q_n) {
-            uint4 o;
-            o.x = __builtin_bit_cast(uint32_t, m1[qb]);
-            o.y = i1[qb];
-            o.z = __builtin_bit_cast(uint32_t, m2[qb]);
-            o.w = i2[qb];
-            a.part[(size_t)slab * a.q_stride + q] = o;
-        }
-    }
-}
 
 struct MergeArgs {
     const uint4* part;
@@ -269,10 +27,12 @@ struct MergeArgs {
     sift_float4* sorted;
     uint32_t* matches;
     float ratio;
+    float* thresh_out;    // seeding mode: write the admission bound instead
 };
 
 // One thread per query: lexicographic merge of the slab partials, then
-// d = key + ||q||^2 and the output formats.
+// d = key + ||q||^2 and the output formats.  In seeding mode the result is
+// the admission bound: the smallest key strictly above the 2nd-best sample key.
 template <int KIND>
 __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
@@ -284,8 +44,19 @@ __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
         const uint4 p = a.part[(size_t)s * a.q_stride + q];
         merge2(m1, i1, m2, i2, __builtin_bit_cast(key_t, p.x), p.y, __builtin_bit_cast(key_t, p.z), p.w);
     }
-    float d1 = FLT_MAX, d2 = FLT_MAX;
     const bool has1 = i1 < a.r_n, has2 = i2 < a.r_n;
+    if (a.thresh_out) {
+        key_t bound = Key<KIND>::none();
+        if (has2) {
+            if constexpr (KIND == KIND_I8)
+                bound = m2 < 0x7FFFFFFF ? m2 + 1 : m2;
+            else
+                bound = next_up(m2);
+        }
+        a.thresh_out[q] = __builtin_bit_cast(float, bound);
+        return;
+    }
+    float d1 = FLT_MAX, d2 = FLT_MAX;
     if constexpr (KIND == KIND_I8) {
         const int nq = __float_as_int(a.q_norm[q]);
         if (has1) d1 = (float)(m1 + nq);
@@ -320,73 +91,99 @@ __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
     }
 }
 
-// Launch shapes per kind.  BF16/F16: 2 query blocks per wave (64 queries),
-// 4 waves, 2 base tiles (64 rows) per LDS stage.
-template <int KIND>
-struct Shape;
-template <> struct Shape<KIND_BF16> { static constexpr int QW = 2, WAVES = 4, TPS = 2; };
-template <> struct Shape<KIND_F16> { static constexpr int QW = 2, WAVES = 4, TPS = 2; };
-template <> struct Shape<KIND_F32> { static constexpr int QW = 2, WAVES = 4, TPS = 1; };
-template <> struct Shape<KIND_I8> { static constexpr int QW = 4, WAVES = 4, TPS = 4; };
+// Base rows scanned by the seeding pass (a bound per query from the first
+// SEED_TILES tiles); used when the base has many more tiles than that.
+constexpr int SEED_TILES = 128;
 
-inline int queries_per_group(int kind) {
-    switch (kind) {
-        case KIND_BF16: return Shape<KIND_BF16>::QW * Shape<KIND_BF16>::WAVES * 32;
-        case KIND_F16: return Shape<KIND_F16>::QW * Shape<KIND_F16>::WAVES * 32;
-        case KIND_F32: return Shape<KIND_F32>::QW * Shape<KIND_F32>::WAVES * 32;
-        default: return Shape<KIND_I8>::QW * Shape<KIND_I8>::WAVES * 32;
-    }
-}
-
-template <int KIND>
-int launch_fused(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
-                 uint32_t r_n, const Knn2Out& out) {
-    using S = Shape<KIND>;
-    const int qpg = S::QW * S::WAVES * 32;
+int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& qs,
+                       const PackedSet& bs, uint32_t q_n, uint32_t r_n, const Knn2Out& out) {
+    const int qpg = sh.qw * sh.waves * 32;
     const int n_groups = (int)((q_n + qpg - 1) / qpg);
+    if ((uint64_t)n_groups * qpg > qs.rows_pad)
+        return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
     const int n_tiles = (int)bs.ntiles;
-    // ~8 resident blocks' worth per CU across 256 CUs, slabs a multiple of 8
-    const int target_blocks = 256 * 8;
-    int want_slabs = std::max(1, std::min(n_tiles, (target_blocks + n_groups - 1) / n_groups));
-    int tiles_per_slab = std::max(1, (n_tiles + want_slabs - 1) / want_slabs);
-    tiles_per_slab = ((tiles_per_slab + S::TPS - 1) / S::TPS) * S::TPS;
-    int n_slabs = std::max(1, (n_tiles + tiles_per_slab - 1) / tiles_per_slab);
-    n_slabs = ((n_slabs + 7) / 8) * 8;
     const uint32_t q_stride = (uint32_t)n_groups * qpg;
-    SIFT_TRY(reserve(h, h->part, (size_t)n_slabs * q_stride * sizeof(uint4)));
+    // ~8 resident blocks per CU over 256 CUs; slabs a multiple of 8 (one XCD each)
+    const int target_blocks = 256 * 8;
+    int want = std::max(1, std::min(n_tiles, (target_blocks + n_groups - 1) / n_groups));
+    int tps_slab = std::max(1, (n_tiles + want - 1) / want);
+    tps_slab = ((tps_slab + sh.tps - 1) / sh.tps) * sh.tps;
+    int n_slabs = std::max(1, (n_tiles + tps_slab - 1) / tps_slab);
+    n_slabs = ((n_slabs + 7) / 8) * 8;
 
-    FusedArgs fa;
-    fa.q_tiles = static_cast<const uint4*>(qs.tiles.p);
-    fa.b_tiles = static_cast<const uint4*>(bs.tiles.p);
-    fa.b_nl = static_cast<const float*>(bs.norm_lane.p);
-    fa.part = static_cast<uint4*>(h->part.p);
-    fa.q_n = q_n;
-    fa.q_stride = q_stride;
-    fa.n_tiles = n_tiles;
-    fa.tiles_per_slab = tiles_per_slab;
-    fa.n_slabs = n_slabs;
-    fa.n_groups = n_groups;
-    const unsigned grid = (unsigned)(n_slabs * n_groups);
+    const bool seed = r_n >= 2 && n_tiles >= 4 * SEED_TILES;
+    const int seed_tiles = SEED_TILES;
+    int seed_slab_tiles = ((seed_tiles / 8 + sh.tps - 1) / sh.tps) * sh.tps;
+    const int seed_slabs = seed ? (((seed_tiles + seed_slab_tiles - 1) / seed_slab_tiles + 7) / 8) * 8 : 0;
+    SIFT_TRY(reserve(h, h->part, (size_t)(n_slabs + seed_slabs) * q_stride * sizeof(uint4)));
+    SIFT_TRY(reserve(h, h->thresh, (size_t)q_stride * sizeof(float)));
+    uint4* part = static_cast<uint4*>(h->part.p);
+    float* thresh = static_cast<float*>(h->thresh.p);
+
+    auto mk_fused = [&](int nt, int per_slab, int slabs, uint4* dst, const float* thr) {
+        FusedArgs fa;
+        fa.q_tiles = static_cast<const uint4*>(qs.tiles.p);
+        fa.b_tiles = static_cast<const uint4*>(bs.tiles.p);
+        fa.b_nl = static_cast<const float*>(bs.norm_lane.p);
+        fa.part = dst;
+        fa.thresh = thr;
+        fa.q_n = q_n;
+        fa.q_stride = q_stride;
+        fa.n_tiles = nt;
+        fa.tiles_per_slab = per_slab;
+        fa.n_slabs = slabs;
+        fa.n_groups = n_groups;
+        return fa;
+    };
+    auto mk_merge = [&](const uint4* src, int slabs) {
+        MergeArgs ma;
+        ma.part = src;
+        ma.q_norm = static_cast<const float*>(qs.norm.p);
+        ma.q_n = q_n;
+        ma.r_n = r_n;
+        ma.q_stride = q_stride;
+        ma.n_slabs = slabs;
+        ma.idx = out.idx;
+        ma.dist = out.dist;
+        ma.sorted = out.sorted;
+        ma.matches = out.matches;
+        ma.ratio = out.ratio;
+        ma.thresh_out = nullptr;
+        return ma;
+    };
+    auto launch_merge = [&](const MergeArgs& ma) {
+        const unsigned g = (q_n + 255) / 256;
+        switch (bs.kind) {
+            case KIND_BF16: merge_top2<KIND_BF16><<<g, 256, 0, h->stream>>>(ma); break;
+            case KIND_F16: merge_top2<KIND_F16><<<g, 256, 0, h->stream>>>(ma); break;
+            case KIND_F32: merge_top2<KIND_F32><<<g, 256, 0, h->stream>>>(ma); break;
+            default: merge_top2<KIND_I8><<<g, 256, 0, h->stream>>>(ma); break;
+        }
+    };
+
+    const float* thr = nullptr;
+    if (seed) {
+        uint4* sp = part + (size_t)n_slabs * q_stride;
+        FusedArgs fa = mk_fused(seed_tiles, seed_slab_tiles, seed_slabs, sp, nullptr);
+        {
+            ProfScope ps(h, PROF_SEED);
+            sh.fn(fa, (unsigned)(seed_slabs * n_groups), h->stream);
+            SIFT_HIP_TRY(h, hipGetLastError());
+            MergeArgs ma = mk_merge(sp, seed_slabs);
+            ma.thresh_out = thresh;
+            launch_merge(ma);
+            SIFT_HIP_TRY(h, hipGetLastError());
+        }
+        thr = thresh;
+    }
     {
         ProfScope ps(h, PROF_FUSED);
-        fused_top2<KIND, S::QW, S::WAVES, S::TPS><<<grid, S::WAVES * 64, 0, h->stream>>>(fa);
+        sh.fn(mk_fused(n_tiles, tps_slab, n_slabs, part, thr), (unsigned)(n_slabs * n_groups), h->stream);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
-    MergeArgs ma;
-    ma.part = static_cast<const uint4*>(h->part.p);
-    ma.q_norm = static_cast<const float*>(qs.norm.p);
-    ma.q_n = q_n;
-    ma.r_n = r_n;
-    ma.q_stride = q_stride;
-    ma.n_slabs = n_slabs;
-    ma.idx = out.idx;
-    ma.dist = out.dist;
-    ma.sorted = out.sorted;
-    ma.matches = out.matches;
-    ma.ratio = out.ratio;
     {
         ProfScope ps(h, PROF_MERGE);
-        merge_top2<KIND><<<(q_n + 255) / 256, 256, 0, h->stream>>>(ma);
+        launch_merge(mk_merge(part, n_slabs));
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     return SIFT_OK;
@@ -398,18 +195,14 @@ int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint3
                 uint32_t r_n, const Knn2Out& out) {
     if (qs.kind != bs.kind) return set_error(h, SIFT_ERR_INVALID, "query/base packed with different kinds");
     if (q_n == 0) return SIFT_OK;
-    switch (bs.kind) {
-        case KIND_BF16: return launch_fused<KIND_BF16>(h, qs, bs, q_n, r_n, out);
-        case KIND_F16: return launch_fused<KIND_F16>(h, qs, bs, q_n, r_n, out);
-        case KIND_F32: return launch_fused<KIND_F32>(h, qs, bs, q_n, r_n, out);
-        case KIND_I8: return launch_fused<KIND_I8>(h, qs, bs, q_n, r_n, out);
-        default: return set_error(h, SIFT_ERR_INVALID, "bad kind");
-    }
+    const FusedShape* sh = pick_shape(bs.kind);
+    if (!sh) return set_error(h, SIFT_ERR_INVALID, "bad kind");
+    return launch_fused_shape(h, *sh, qs, bs, q_n, r_n, out);
 }
 
 uint32_t query_rows_pad(int kind, uint32_t q_n) {
-    const uint32_t qpg = (uint32_t)queries_per_group(kind);
-    return std::max<uint32_t>(1, (q_n + qpg - 1) / qpg) * qpg;
+    (void)kind;
+    return std::max<uint32_t>(1, (q_n + QPAD - 1) / QPAD) * QPAD;
 }
 
 }  // namespace sift
@@ -484,7 +277,7 @@ sift_context* g_default[64] = {};
 
 sift_context* default_handle() {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     if (dev < 0 || dev >= 64) return nullptr;
     std::lock_guard<std::mutex> lk(g_default_mu);
     if (!g_default[dev]) sift_create(&g_default[dev]);
@@ -648,7 +441,7 @@ int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, 
     ix->kind = kind;
     SIFT_TRY(reserve(h, h->flag, 16));
     int* flag = static_cast<int*>(h->flag.p);
-    hipMemsetAsync(flag, 0, 4, h->stream);
+    (void)hipMemsetAsync(flag, 0, 4, h->stream);
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
     int rc = pack_rows(h, r, type, r_n, rows_b, kind, true, ix->base, flag);
     if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess)
@@ -682,7 +475,7 @@ int sift_bf_index_query(sift_handle_t h, sift_bf_index_t ix, const void* q, int 
 
 int sift_bf_index_destroy(sift_bf_index_t ix) {
     if (!ix) return SIFT_OK;
-    hipStreamSynchronize(ix->h->stream);
+    (void)hipStreamSynchronize(ix->h->stream);
     release(ix->base.tiles);
     release(ix->base.norm);
     release(ix->base.norm_lane);

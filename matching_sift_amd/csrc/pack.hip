@@ -92,6 +92,9 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
 #pragma unroll
         for (int i = 0; i < 8; i++) v[i] = (KIND == KIND_I8) ? 128.f : 0.f;
     }
+    // Query operands of the float kinds are stored as -2q (exact: a power-of-two
+    // scale); the norm below is computed from q itself.
+    const float opscale = (!is_base && KIND != KIND_I8) ? -2.f : 1.f;
     const uint32_t t = row >> 5, i32 = row & 31;
     constexpr int G = (KIND == KIND_F32) ? 16 : (KIND == KIND_I8 ? 4 : 8);
     uint4* tile = tiles + (size_t)t * G * 64;
@@ -105,7 +108,27 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
             w[i] = lo | (hi << 16);
         }
         const int g = blk >> 1, h = blk & 1;
-        tile[g * 64 + i32 + 32 * h] = make_uint4(w[0], w[1], w[2], w[3]);
+        if (opscale != 1.f) {
+            uint32_t o[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float lo, hi;
+                if constexpr (KIND == KIND_BF16) {
+                    lo = __uint_as_float(w[i] << 16);
+                    hi = __uint_as_float(w[i] & 0xFFFF0000u);
+                    o[i] = f2bf(opscale * lo) | (f2bf(opscale * hi) << 16);
+                } else {
+                    _Float16 a, b;
+                    uint16_t la = (uint16_t)(w[i] & 0xFFFF), hb = (uint16_t)(w[i] >> 16);
+                    __builtin_memcpy(&a, &la, 2);
+                    __builtin_memcpy(&b, &hb, 2);
+                    o[i] = f2h(opscale * (float)a) | (f2h(opscale * (float)b) << 16);
+                }
+            }
+            tile[g * 64 + i32 + 32 * h] = make_uint4(o[0], o[1], o[2], o[3]);
+        } else {
+            tile[g * 64 + i32 + 32 * h] = make_uint4(w[0], w[1], w[2], w[3]);
+        }
         // norm of the rounded values the MFMA sees (identical for integer data)
         float s = 0.f;
 #pragma unroll
@@ -128,10 +151,10 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
         nrm = s;
     } else if constexpr (KIND == KIND_F32) {
         // group g = blk holds k-steps 4g..4g+3; lane half h holds dims 8g + 2i + h
-        tile[blk * 64 + i32] = make_uint4(__float_as_uint(v[0]), __float_as_uint(v[2]),
-                                          __float_as_uint(v[4]), __float_as_uint(v[6]));
-        tile[blk * 64 + i32 + 32] = make_uint4(__float_as_uint(v[1]), __float_as_uint(v[3]),
-                                               __float_as_uint(v[5]), __float_as_uint(v[7]));
+        tile[blk * 64 + i32] = make_uint4(__float_as_uint(opscale * v[0]), __float_as_uint(opscale * v[2]),
+                                          __float_as_uint(opscale * v[4]), __float_as_uint(opscale * v[6]));
+        tile[blk * 64 + i32 + 32] = make_uint4(__float_as_uint(opscale * v[1]), __float_as_uint(opscale * v[3]),
+                                               __float_as_uint(opscale * v[5]), __float_as_uint(opscale * v[7]));
         float s = 0.f;
 #pragma unroll
         for (int i = 0; i < 8; i++) s = fmaf(v[i], v[i], s);

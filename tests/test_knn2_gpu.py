@@ -169,3 +169,30 @@ def test_unit_norm_float_data_f32(handle):
     assert agree >= 0.99
     ok = srt[:, 2] == s[:, 2]
     np.testing.assert_allclose(srt[ok, 0], s[ok, 0], rtol=1e-5, atol=1e-6)
+
+
+SHAPES = {COMPUTE_BF16: ["2,4,2", "2,4,4", "2,8,2", "4,4,2", "1,8,4"],
+          COMPUTE_F16: ["2,4,2"], COMPUTE_F32: ["2,4,1"], COMPUTE_I8: ["4,4,4", "2,4,4", "4,8,4"]}
+
+
+@pytest.mark.parametrize("compute,shape", [(c, s) for c in SHAPES for s in SHAPES[c]])
+def test_seeded_bound_all_shapes(handle, compute, shape, monkeypatch):
+    """Bases above 16,384 rows take the seeded-bound path (a per-query bound
+    from the first 4,096 rows); exercise it with realistic data and with a
+    3-letter alphabet where many rows tie exactly with the bound."""
+    monkeypatch.setenv("SIFT_SHAPE", shape)
+    say("seeded", compute, shape)
+    b = datagen.base(40_000, seed=71)
+    q = datagen.queries(b, 300, seed=72)
+    s, mt = expected(q, b)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+    rng = np.random.default_rng(5)
+    r = rng.integers(0, 3, size=(20_000, 128)).astype(np.float32)
+    r[17_000:17_100] = r[100:200]  # duplicates of sample rows far beyond the sample
+    qq = rng.integers(0, 3, size=(150, 128)).astype(np.float32)
+    qq[:50] = r[100:150]
+    s, mt = expected(qq, r)
+    idx, dist, srt, mat = knn2(handle, qq, r, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
