@@ -63,8 +63,9 @@ enum {
 
 /* Hash kinds for the LSH index (lsh_type in the reference, src/lsh_h.cu:716-718). */
 enum {
-    SIFT_LSH_HYPERPLANE = 0, /* random hyperplanes (reference DOT_INT_BUCKET) */
-    SIFT_LSH_LEARNED = 1     /* learned 128->256->b MLP hash                  */
+    SIFT_LSH_HYPERPLANE = 0,     /* Gaussian hyperplanes (reference DOT_INT_BUCKET) */
+    SIFT_LSH_LEARNED = 1,        /* learned 128->256->b MLP hash                    */
+    SIFT_LSH_HYPERPLANE_PM1 = 2  /* +-1 hyperplanes as in src/lsh_h.cu:51-84        */
 };
 
 enum {
@@ -174,6 +175,31 @@ int sift_bf_index_query(sift_handle_t handle, sift_bf_index_t index, const void*
                         uint32_t q_n, uint32_t* idx, float* dist, sift_float4* sorted,
                         uint32_t* matches, float ratio, int blocking);
 int sift_bf_index_destroy(sift_bf_index_t index);
+
+/* LSH index.  Replaces the per-call table build of lsh_gpu
+ * (src/lsh_h.cu:903-914) and lsh_thread_sort_reduce (include/lsh_h.h:66,
+ * src/lsh_h.cu:565): L tables of nbits-bit sign codes (bit b set when
+ * h_b . x < 0), all tables' (code, row) pairs radix-sorted on the device.
+ * Descriptors must hold integers in [0,255] (SIFT_ERR_NOT_EXACT otherwise).
+ * nbits in [1,32], L in [1,256], nbits + ceil(log2 L) <= 32. */
+typedef struct sift_lsh_index* sift_lsh_index_t;
+int sift_lsh_build(sift_handle_t handle, const void* r, int type, uint32_t r_n, int nbits, int L,
+                   int hash_kind, uint64_t seed, sift_lsh_index_t* index);
+/* Exact int32 re-rank of the union of the query's buckets over all tables.
+ * Outputs as sift_knn2 plus ncand[q_n] = candidates examined (with repeats
+ * across tables); any output may be NULL. */
+int sift_lsh_query(sift_handle_t handle, sift_lsh_index_t index, const void* q, int type,
+                   uint32_t q_n, uint32_t* idx, float* dist, uint32_t* ncand, uint32_t* matches,
+                   float ratio, int blocking);
+int sift_lsh_destroy(sift_lsh_index_t index);
+/* Query codes, device uint32[L][q_n] (blocking). */
+int sift_lsh_codes(sift_handle_t handle, sift_lsh_index_t index, const void* q, int type,
+                   uint32_t q_n, uint32_t* codes);
+int sift_lsh_info(sift_lsh_index_t index, uint32_t* r_n, int* nbits, int* L);
+/* Host copies: hyperplanes float[L][nbits][128]; sorted keys (t << nbits | code)
+ * and base rows, uint32[L*r_n] each. */
+int sift_lsh_get_hyperplanes(sift_lsh_index_t index, float* planes);
+int sift_lsh_get_buckets(sift_lsh_index_t index, uint32_t* keys, uint32_t* rows);
 
 #ifdef __cplusplus
 }

@@ -10,9 +10,9 @@ from ._lib import (  # noqa: F401
     LIB_PATH, SiftError, lib, load,
     FLOAT_HOST, FLOAT_DEVICE, HALF_DEVICE, BF16_DEVICE, U8_DEVICE,
     COMPUTE_AUTO, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8, COMPUTE_FP8,
-    LSH_HYPERPLANE, LSH_LEARNED,
+    LSH_HYPERPLANE, LSH_LEARNED, LSH_HYPERPLANE_PM1,
 )
 from .api import (  # noqa: F401
     DeviceArray, Handle, to_device, knn2, cublas_2nn_sift, device_brute, cublas_2nn_f,
-    BruteForceIndex,
+    BruteForceIndex, LSHIndex, lsh_gpu,
 )

@@ -7,7 +7,7 @@ LIB_PATH = os.path.join(_PKG, "libsift2nn.so")
 
 FLOAT_HOST, FLOAT_DEVICE, HALF_DEVICE, BF16_DEVICE, U8_DEVICE = 0, 1, 2, 3, 4
 COMPUTE_AUTO, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8, COMPUTE_FP8 = 0, 1, 2, 3, 4, 5
-LSH_HYPERPLANE, LSH_LEARNED = 0, 1
+LSH_HYPERPLANE, LSH_LEARNED, LSH_HYPERPLANE_PM1 = 0, 1, 2
 
 ERRORS = {-1: "invalid argument", -2: "out of device memory", -3: "HIP runtime error",
           -4: "unsupported", -5: "data outside the exact domain"}
@@ -54,6 +54,13 @@ SIGNATURES = {
     "sift_bf_index_build": (i32, [vp, vp, i32, u32, i32, C.POINTER(vp)]),
     "sift_bf_index_query": (i32, [vp, vp, vp, i32, u32, vp, vp, vp, vp, f32, i32]),
     "sift_bf_index_destroy": (i32, [vp]),
+    "sift_lsh_build": (i32, [vp, vp, i32, u32, i32, i32, i32, u64, C.POINTER(vp)]),
+    "sift_lsh_query": (i32, [vp, vp, vp, i32, u32, vp, vp, vp, vp, f32, i32]),
+    "sift_lsh_destroy": (i32, [vp]),
+    "sift_lsh_codes": (i32, [vp, vp, vp, i32, u32, vp]),
+    "sift_lsh_info": (i32, [vp, C.POINTER(u32), C.POINTER(i32), C.POINTER(i32)]),
+    "sift_lsh_get_hyperplanes": (i32, [vp, vp]),
+    "sift_lsh_get_buckets": (i32, [vp, vp, vp]),
 }
 
 

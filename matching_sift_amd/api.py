@@ -170,6 +170,64 @@ class BruteForceIndex:
             pass
 
 
+class LSHIndex:
+    """Random-hyperplane LSH index (sift_lsh_build / sift_lsh_query)."""
+
+    def __init__(self, handle, r, nbits, L, hash_kind=_lib.LSH_HYPERPLANE, seed=0, type_=None):
+        self.handle = handle
+        rd = to_device(r)
+        self._keep = rd
+        self.n, self.nbits, self.L = rd.shape[0], int(nbits), int(L)
+        ix = C.c_void_p()
+        check(lib().sift_lsh_build(handle.h, rd.ptr, _input_type(rd, type_), self.n, self.nbits,
+                                   self.L, hash_kind, int(seed), C.byref(ix)), handle.h)
+        self.ix = ix.value
+
+    def query(self, q, ratio=0.8, type_=None, outputs=None, blocking=True):
+        qd = to_device(q)
+        M = qd.shape[0]
+        if outputs is None:
+            outputs = (DeviceArray.empty((M, 2), np.uint32), DeviceArray.empty((M, 2), np.float32),
+                       DeviceArray.empty((M,), np.uint32), DeviceArray.empty((M,), np.uint32))
+        idx, dist, nc, mat = outputs
+        ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
+        check(lib().sift_lsh_query(self.handle.h, self.ix, qd.ptr, _input_type(qd, type_), M,
+                                   ptr(idx), ptr(dist), ptr(nc), ptr(mat), float(ratio),
+                                   1 if blocking else 0), self.handle.h)
+        if not blocking:
+            return outputs
+        return tuple(o.numpy() if o is not None else None for o in outputs)
+
+    def codes(self, q, type_=None):
+        qd = to_device(q)
+        out = DeviceArray.empty((self.L, qd.shape[0]), np.uint32)
+        check(lib().sift_lsh_codes(self.handle.h, self.ix, qd.ptr, _input_type(qd, type_),
+                                   qd.shape[0], out.ptr), self.handle.h)
+        return out.numpy()
+
+    def hyperplanes(self):
+        out = np.empty((self.L, self.nbits, 128), np.float32)
+        check(lib().sift_lsh_get_hyperplanes(self.ix, out.ctypes.data))
+        return out
+
+    def buckets(self):
+        keys = np.empty((self.L * self.n,), np.uint32)
+        rows = np.empty((self.L * self.n,), np.uint32)
+        check(lib().sift_lsh_get_buckets(self.ix, keys.ctypes.data, rows.ctypes.data))
+        return keys, rows
+
+    def close(self):
+        if self.ix:
+            lib().sift_lsh_destroy(self.ix)
+            self.ix = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------- reference-named ----
 
 def cublas_2nn_sift(q_points, r_points, type_, q_n, r_n, matches, threshold, handle,
@@ -190,3 +248,10 @@ def cublas_2nn_f(q_points, r_points, q_n, r_n, sorted_, handle):
     """src/knn_brute.cu:523."""
     lib().cublas_2nn_f(q_points.ptr, r_points.ptr, q_n, r_n, sorted_.ptr, handle.h)
     return check(lib().sift_last_status(), handle.h)
+
+
+def lsh_gpu(q_points, r_points, type_, q_n, r_n, matches, threshold, handle, stream_n, l,
+            lsh_type, nbits):
+    """src/lsh_h.cu:722 — build l tables of nbits bits, query, ratio-test."""
+    return check(lib().lsh_gpu(q_points.ptr, r_points.ptr, type_, q_n, r_n, matches.ptr,
+                               float(threshold), handle.h, stream_n, l, lsh_type, nbits), handle.h)

@@ -82,6 +82,15 @@ void prof_flush(sift_context* h) {
     h->npend = 0;
 }
 
+bool device_accessible(const void* p) {
+    hipPointerAttribute_t attr;
+    const bool ok = hipPointerGetAttributes(&attr, p) == hipSuccess &&
+                    (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+                     attr.type == hipMemoryTypeHost);
+    (void)hipGetLastError();
+    return ok;
+}
+
 size_t input_row_bytes(int type) {
     switch (type) {
         case SIFT_FLOAT_HOST:

@@ -17,7 +17,7 @@ struct KT;
 template <>
 struct KT<KIND_BF16> {
     using acc_t = f32x16;
-    static constexpr int G = 8;
+    static constexpr int G = 9;  // 8 data groups + the norm group
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -26,7 +26,7 @@ struct KT<KIND_BF16> {
 template <>
 struct KT<KIND_F16> {
     using acc_t = f32x16;
-    static constexpr int G = 8;
+    static constexpr int G = 9;
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
@@ -35,7 +35,7 @@ struct KT<KIND_F16> {
 template <>
 struct KT<KIND_F32> {
     using acc_t = f32x16;
-    static constexpr int G = 16;
+    static constexpr int G = 17;
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.x), __uint_as_float(b.x), c, 0, 0, 0);
         c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.y), __uint_as_float(b.y), c, 0, 0, 0);
@@ -101,10 +101,11 @@ __device__ inline void merge2(T& a1, uint32_t& ia1, T& a2, uint32_t& ia2, T b1, 
 struct FusedArgs {
     const uint4* q_tiles;   // packed queries, [qblocks][G][64]
     const uint4* b_tiles;   // packed base, [n_tiles][G][64]
-    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32]
+    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32] (I8 only)
     uint4* part;            // [n_slabs][q_stride] {key1, idx1, key2, idx2}
     const float* thresh;    // per-query admission bound (key bits) or null
     uint32_t q_n, q_stride;
+    uint32_t r_n;           // real base rows; rows >= r_n in the last tile are masked
     int n_tiles, tiles_per_slab, n_slabs, n_groups;
 };
 

@@ -41,15 +41,16 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     using acc_t = typename KTk::acc_t;
     using key_t = typename Key<KIND>::T;
     constexpr int G = KTk::G;
-    constexpr int UPT = G * 64;               // 16-byte units per tile
+    constexpr bool NORM_IN_K = KIND != KIND_I8;  // float kinds: |b|^2 comes out of the MFMA
+    constexpr int UPT = G * 64;                   // 16-byte units per tile
     constexpr int NT = WAVES * 64;
     constexpr int STAGE = TPS * UPT;
-    constexpr int PER_THREAD = STAGE / NT;
-    static_assert(STAGE % NT == 0, "stage must split evenly over the block");
+    constexpr int PER_THREAD = (STAGE + NT - 1) / NT;
     static_assert(TPS * 32 <= NT, "one norm per thread per stage");
 
-    __shared__ __align__(16) uint4 s_tiles[STAGE];
-    __shared__ __align__(16) float s_nl[TPS * 32];
+    // two LDS stages: stage s+1 is written while stage s is consumed
+    __shared__ __align__(16) uint4 s_tiles[2][STAGE];
+    __shared__ __align__(16) float s_nl[2][NORM_IN_K ? 1 : TPS * 32];
 
     const int b = blockIdx.x;
     const int xcd = b & 7, r = b >> 3;
@@ -87,96 +88,117 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     uint4 pre[PER_THREAD];
     float pre_nl = 0.f;
     auto load_stage = [&](int ts) {
-        const int avail = min(TPS, t_end - ts);
+        const int avail = min(TPS, t_end - ts) * UPT;
         const uint4* src = a.b_tiles + (size_t)ts * UPT;
 #pragma unroll
         for (int u = 0; u < PER_THREAD; u++) {
             const int idx = u * NT + tid;
-            pre[u] = (idx < avail * UPT) ? src[idx] : make_uint4(0u, 0u, 0u, 0u);
+            pre[u] = (idx < avail) ? src[idx] : make_uint4(0u, 0u, 0u, 0u);
         }
-        if (tid < TPS * 32) pre_nl = (tid < avail * 32) ? a.b_nl[(size_t)ts * 32 + tid] : 0.f;
+        if constexpr (!NORM_IN_K)
+            if (tid < TPS * 32) pre_nl = (tid < min(TPS, t_end - ts) * 32) ? a.b_nl[(size_t)ts * 32 + tid] : 0.f;
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < PER_THREAD; u++) {
+            const int idx = u * NT + tid;
+            if (PER_THREAD * NT == STAGE || idx < STAGE) s_tiles[buf][idx] = pre[u];
+        }
+        if constexpr (!NORM_IN_K)
+            if (tid < TPS * 32) s_nl[buf][tid] = pre_nl;
     };
 
-    if (t_begin < t_end) load_stage(t_begin);
+    if (t_begin < t_end) {
+        load_stage(t_begin);
+        store_stage(0);
+    }
+    __syncthreads();
+    int buf = 0;
     for (int ts = t_begin; ts < t_end; ts += TPS) {
-        __syncthreads();  // previous stage fully consumed
-#pragma unroll
-        for (int u = 0; u < PER_THREAD; u++) s_tiles[u * NT + tid] = pre[u];
-        if (tid < TPS * 32) s_nl[tid] = pre_nl;
-        __syncthreads();
-        if (ts + TPS < t_end) load_stage(ts + TPS);  // in flight during this stage
+        const bool more = ts + TPS < t_end;
+        if (more) load_stage(ts + TPS);  // in flight while this stage is consumed
         const int ntile = min(TPS, t_end - ts);
-        // Register double buffer: the A fragments and norms of tile tt+1 are
-        // read from LDS while tile tt's MFMAs run.
-        uint4 afr[2][G];
-        float nlr[2][16];
-        auto read_tile = [&](int tt, uint4* af, float* nl) {
-#pragma unroll
-            for (int g = 0; g < G; g++) af[g] = s_tiles[tt * UPT + g * 64 + lane];
-            const float4* nlp = reinterpret_cast<const float4*>(s_nl + tt * 32 + h * 16);
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const float4 v = nlp[c];
-                nl[4 * c] = v.x; nl[4 * c + 1] = v.y; nl[4 * c + 2] = v.z; nl[4 * c + 3] = v.w;
-            }
-        };
-        read_tile(0, afr[0], nlr[0]);
 #pragma unroll
         for (int tt = 0; tt < TPS; tt++) {
             if (tt >= ntile) break;
-            const int cur = tt & 1;
-            if (tt + 1 < ntile) read_tile(tt + 1, afr[cur ^ 1], nlr[cur ^ 1]);
-            // Float kinds: queries are packed as -2q and the accumulator starts
-            // at the row norms, so the MFMA chain yields t = |b|^2 - 2 q.b
-            // directly (every partial sum is an integer below 2^24 in
-            // magnitude for SIFT data, hence exact).  I8: t = |b'|^2 - 2 acc.
+            const uint4* tp = &s_tiles[buf][tt * UPT];
             acc_t acc[QW];
 #pragma unroll
-            for (int qb = 0; qb < QW; qb++) {
-#pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    if constexpr (KIND == KIND_I8)
-                        acc[qb][j] = 0;
-                    else
-                        acc[qb][j] = nlr[cur][j];
-                }
-            }
+            for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
 #pragma unroll
             for (int g = 0; g < G; g++) {
+                const uint4 av = tp[g * 64 + lane];
 #pragma unroll
-                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[cur][g], qop[qb][g], acc[qb]);
+                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(av, qop[qb][g], acc[qb]);
             }
-            const uint32_t rowbase = (uint32_t)(ts + tt) * 32u + 4u * h;
+            const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
+            const uint32_t rowbase = tile_row0 + 4u * h;
+            // keys: float kinds t = |b|^2 - 2 q.b straight from the MFMA (the
+            // queries are packed as -2q, the norm group adds |b|^2; all partial
+            // sums are integers below 2^24 for SIFT data, hence exact);
+            // I8: t = |b'|^2 - 2 q'.b' with the norms from LDS
+            key_t tv[QW][16];
+            if constexpr (NORM_IN_K) {
 #pragma unroll
-            for (int qb = 0; qb < QW; qb++) {
-                key_t tv[16];
+                for (int qb = 0; qb < QW; qb++)
 #pragma unroll
-                for (int j = 0; j < 16; j++) {
-                    if constexpr (KIND == KIND_I8)
-                        tv[j] = __float_as_int(nlr[cur][j]) - 2 * acc[qb][j];
-                    else
-                        tv[j] = acc[qb][j];
-                }
-                // registers 4c..4c+3 hold 4 consecutive rows (8c + 4h + e):
-                // one admission test per group keeps the divergent insert short
+                    for (int j = 0; j < 16; j++) tv[qb][j] = acc[qb][j];
+                if (tile_row0 + 32u > a.r_n) {  // last tile: pad rows never rank
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const key_t gm = min(min(tv[4 * c], tv[4 * c + 1]), min(tv[4 * c + 2], tv[4 * c + 3]));
-                    if (gm < m2[qb]) {  // rare: a row beats this lane's 2nd best
+                    for (int j = 0; j < 16; j++) {
+                        const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= a.r_n;
 #pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            const key_t v = tv[4 * c + e];
-                            const uint32_t row = rowbase + 8u * c + e;
-                            const bool c1 = v < m1[qb], c2 = v < m2[qb];
-                            m2[qb] = c1 ? m1[qb] : (c2 ? v : m2[qb]);
-                            i2[qb] = c1 ? i1[qb] : (c2 ? row : i2[qb]);
-                            m1[qb] = c1 ? v : m1[qb];
-                            i1[qb] = c1 ? row : i1[qb];
-                        }
+                        for (int qb = 0; qb < QW; qb++) tv[qb][j] = pad ? Key<KIND>::none() : tv[qb][j];
                     }
                 }
+            } else {
+                const float4* nlp = reinterpret_cast<const float4*>(&s_nl[buf][tt * 32 + h * 16]);
+                int nl[16];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float4 v = nlp[c];
+                    nl[4 * c] = __float_as_int(v.x); nl[4 * c + 1] = __float_as_int(v.y);
+                    nl[4 * c + 2] = __float_as_int(v.z); nl[4 * c + 3] = __float_as_int(v.w);
+                }
+#pragma unroll
+                for (int qb = 0; qb < QW; qb++)
+#pragma unroll
+                    for (int j = 0; j < 16; j++) tv[qb][j] = nl[j] - 2 * acc[qb][j];
+            }
+            // admission: registers 4c..4c+3 hold 4 consecutive rows (8c + 4h + e);
+            // one wave-wide branch per tile, per-group tests only inside it
+            key_t gm[QW][4];
+            bool any = false;
+#pragma unroll
+            for (int qb = 0; qb < QW; qb++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    gm[qb][c] = min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3]));
+                    any |= gm[qb][c] < m2[qb];
+                }
+            if (any) {  // rare once the bound is tight
+#pragma unroll
+                for (int qb = 0; qb < QW; qb++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (gm[qb][c] < m2[qb]) {
+#pragma unroll
+                            for (int e = 0; e < 4; e++) {
+                                const key_t v = tv[qb][4 * c + e];
+                                const uint32_t row = rowbase + 8u * c + e;
+                                const bool c1 = v < m1[qb], c2 = v < m2[qb];
+                                m2[qb] = c1 ? m1[qb] : (c2 ? v : m2[qb]);
+                                i2[qb] = c1 ? i1[qb] : (c2 ? row : i2[qb]);
+                                m1[qb] = c1 ? v : m1[qb];
+                                i1[qb] = c1 ? row : i1[qb];
+                            }
+                        }
+                    }
             }
         }
+        if (more) store_stage(buf ^ 1);  // the other buffer was released by the last barrier
+        __syncthreads();
+        buf ^= 1;
     }
 
     // lanes l and l^32 hold disjoint rows of the same query
@@ -198,7 +220,6 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
         }
     }
 }
-
 
 template <int KIND, int QW, int WAVES, int TPS>
 void fused_launch(const FusedArgs& a, unsigned grid, hipStream_t s) {

@@ -50,11 +50,14 @@ struct ProfScope {
 void prof_flush(sift_context* h);  // accumulate completed event pairs (syncs)
 
 // Packed operand layout: a "tile" is 32 rows of 128 dims stored in MFMA
-// operand order, [group][64 lanes][16 bytes]; lane l holds row l%32.
+// operand order, [group][64 lanes][16 bytes]; lane l holds row l%32.  The
+// float kinds append one "norm group" (K = 128 + 16): base rows carry their
+// squared norm there and query rows the matching multiplier, so the MFMA
+// itself adds |b|^2 to -2 q.b.
 enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4 };
 constexpr int TILE_ROWS = 32;
 inline int groups_per_tile(int kind) {
-    return kind == KIND_F32 ? 16 : (kind == KIND_I8 ? 4 : 8);
+    return kind == KIND_F32 ? 17 : (kind == KIND_I8 ? 4 : 9);
 }
 inline size_t tile_bytes(int kind) { return (size_t)groups_per_tile(kind) * 64 * 16; }
 
@@ -67,10 +70,17 @@ struct PackedSet {
 };
 
 // Pack n rows of `type` input into `kind` tiles padded to rows_pad rows.
-// Pad rows of a base set get norm = +inf (INT32_MAX for I8) so they never rank.
+// PACK_BASE: also writes norm_lane; pad rows get norm = +inf (INT32_MAX for
+//            I8) so they never rank.
+// PACK_QUERY: float kinds store -2q (the fused kernel's operand).
+// PACK_PLAIN: values as given (hyperplanes, projection inputs).
 // `bad` (device int) is OR-ed with 1 if a value is not an integer in [0,255].
+enum PackRole { PACK_PLAIN = 0, PACK_BASE = 1, PACK_QUERY = 2 };
 int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
-              bool is_base, PackedSet& out, int* bad_dev);
+              int role, PackedSet& out, int* bad_dev);
+// uint8 copy [n][128] and int32 squared norms of n input rows (flags non-integers).
+int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
+          int* bad_dev);
 // Check that every value is an integer in [0,255]; result in *bad_dev (OR-ed).
 int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_dev);
 
@@ -89,6 +99,8 @@ int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint3
 int stage_host_input(sift_context* h, const void* host, uint32_t n, DevBuf& buf,
                      const void** dev);
 size_t input_row_bytes(int type);
+// True for device, managed and pinned (mapped) host memory.
+bool device_accessible(const void* p);
 
 }  // namespace sift
 

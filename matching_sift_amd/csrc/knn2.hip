@@ -128,6 +128,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         fa.part = dst;
         fa.thresh = thr;
         fa.q_n = q_n;
+        fa.r_n = r_n;
         fa.q_stride = q_stride;
         fa.n_tiles = nt;
         fa.tiles_per_slab = per_slab;
@@ -259,8 +260,8 @@ int knn2_impl(sift_context* h, const void* q, const void* r, int type, uint32_t 
     int* flag = static_cast<int*>(h->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 4, h->stream));
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    SIFT_TRY(pack_rows(h, r, type, r_n, rows_b, kind, true, h->b_set, flag));
-    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(kind, q_n), kind, false, h->q_set, flag));
+    SIFT_TRY(pack_rows(h, r, type, r_n, rows_b, kind, PACK_BASE, h->b_set, flag));
+    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(kind, q_n), kind, PACK_QUERY, h->q_set, flag));
     SIFT_TRY(knn2_packed(h, h->q_set, h->b_set, q_n, r_n, out));
     if (blocking) {
         int bad = 0;
@@ -318,12 +319,7 @@ int cublas_2nn_sift(void* q_points, void* r_points, int type, uint32_t q_n, uint
     o.ratio = threshold;
     // host-memory matches (FLOAT_HOST callers) go through a device buffer
     if (type == SIFT_FLOAT_HOST && handle) {
-        hipPointerAttribute_t attr;
-        bool device_ok = hipPointerGetAttributes(&attr, matches) == hipSuccess &&
-                         (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
-                          attr.type == hipMemoryTypeHost);
-        (void)hipGetLastError();
-        if (!device_ok) {
+        if (!device_accessible(matches)) {
             DevBuf tmp;
             SIFT_TRY(reserve(handle, tmp, (size_t)q_n * 4));
             o.matches = static_cast<uint32_t*>(tmp.p);
@@ -358,8 +354,8 @@ int cublas_2nn_sift_batch(void* q_points, void* Q, int type, void* R, uint32_t q
     SIFT_TRY(reserve(handle, handle->flag, 16));
     int* flag = static_cast<int*>(handle->flag.p);
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    SIFT_TRY(pack_rows(handle, R, SIFT_HALF_DEVICE, r_n, rows_b, KIND_F16, true, handle->b_set, flag));
-    SIFT_TRY(pack_rows(handle, q_points, type, q_n, query_rows_pad(KIND_F16, q_n), KIND_F16, false,
+    SIFT_TRY(pack_rows(handle, R, SIFT_HALF_DEVICE, r_n, rows_b, KIND_F16, PACK_BASE, handle->b_set, flag));
+    SIFT_TRY(pack_rows(handle, q_points, type, q_n, query_rows_pad(KIND_F16, q_n), KIND_F16, PACK_QUERY,
                        handle->q_set, flag));
     (void)compute;
     Knn2Out o;
@@ -443,7 +439,7 @@ int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, 
     int* flag = static_cast<int*>(h->flag.p);
     (void)hipMemsetAsync(flag, 0, 4, h->stream);
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    int rc = pack_rows(h, r, type, r_n, rows_b, kind, true, ix->base, flag);
+    int rc = pack_rows(h, r, type, r_n, rows_b, kind, PACK_BASE, ix->base, flag);
     if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess)
         rc = set_error(h, SIFT_ERR_HIP, "index build sync");
     if (rc != SIFT_OK) {
@@ -466,7 +462,7 @@ int sift_bf_index_query(sift_handle_t h, sift_bf_index_t ix, const void* q, int 
     if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
     SIFT_TRY(reserve(h, h->flag, 16));
     int* flag = static_cast<int*>(h->flag.p);
-    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, false, h->q_set, flag));
+    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, h->q_set, flag));
     Knn2Out o{idx, dist, sorted, matches, ratio};
     SIFT_TRY(knn2_packed(h, h->q_set, ix->base, q_n, ix->r_n, o));
     if (blocking) SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));

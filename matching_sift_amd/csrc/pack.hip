@@ -74,8 +74,9 @@ template <int TYPE, int KIND>
 __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, uint32_t n,
                                                    uint32_t rows_pad, uint4* __restrict__ tiles,
                                                    float* __restrict__ norm,
-                                                   float* __restrict__ norm_lane, int is_base,
+                                                   float* __restrict__ norm_lane, int role,
                                                    int* __restrict__ bad) {
+    const bool is_base = role == PACK_BASE;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t row = gid >> 4;
     const int blk = gid & 15;
@@ -94,9 +95,9 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
     }
     // Query operands of the float kinds are stored as -2q (exact: a power-of-two
     // scale); the norm below is computed from q itself.
-    const float opscale = (!is_base && KIND != KIND_I8) ? -2.f : 1.f;
+    const float opscale = (role == PACK_QUERY && KIND != KIND_I8) ? -2.f : 1.f;
     const uint32_t t = row >> 5, i32 = row & 31;
-    constexpr int G = (KIND == KIND_F32) ? 16 : (KIND == KIND_I8 ? 4 : 8);
+    constexpr int G = (KIND == KIND_F32) ? 17 : (KIND == KIND_I8 ? 4 : 9);
     uint4* tile = tiles + (size_t)t * G * 64;
     float nrm;
     if constexpr (KIND == KIND_BF16 || KIND == KIND_F16) {
@@ -198,6 +199,44 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
             norm_lane[(size_t)t * 32 + hl * 16 + j] = nrm;
         }
     }
+    // Norm group (float kinds): threads 0/1 of the row write lane halves 0/1.
+    //   base  : |b|^2 (F32), or |b|^2 / 256 split into three parts (BF16/F16,
+    //           exact for integer norms below 2^24)
+    //   query : the multiplier (1, or 256 per part) so the MFMA adds |b|^2
+    //   pads and plain rows: zero
+    if constexpr (KIND != KIND_I8) {
+        if (blk < 2) {
+            uint4 u = make_uint4(0u, 0u, 0u, 0u);
+            if (blk == 0 && (role == PACK_QUERY || (role == PACK_BASE && real))) {
+                if constexpr (KIND == KIND_F32) {
+                    u.x = __float_as_uint(role == PACK_QUERY ? 1.f : nrm);
+                } else if (role == PACK_QUERY) {
+                    const uint32_t m = (KIND == KIND_BF16) ? f2bf(256.f) : f2h(256.f);
+                    u.x = m | (m << 16);
+                    u.y = m;
+                } else {
+                    float v = nrm * (1.f / 256.f);
+                    uint32_t p[3];
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        if constexpr (KIND == KIND_BF16) {
+                            p[i] = f2bf(v);
+                            v -= __uint_as_float(p[i] << 16);
+                        } else {
+                            p[i] = f2h(v);
+                            _Float16 hv;
+                            uint16_t hb = (uint16_t)p[i];
+                            __builtin_memcpy(&hv, &hb, 2);
+                            v -= (float)hv;
+                        }
+                    }
+                    u.x = p[0] | (p[1] << 16);
+                    u.y = p[2];
+                }
+            }
+            tile[(G - 1) * 64 + i32 + 32 * blk] = u;
+        }
+    }
 }
 
 template <int TYPE>
@@ -215,9 +254,40 @@ __global__ __launch_bounds__(256) void check_kernel(const void* __restrict__ in,
     flag_wave(!ok, bad);
 }
 
+// Compact copy for gathers: uint8 rows and int32 squared norms (16 threads
+// per row, 8 dims each).  Values must be integers in [0,255] (flagged).
+template <int TYPE>
+__global__ __launch_bounds__(256) void to_u8_kernel(const void* __restrict__ in, uint32_t n,
+                                                    uint2* __restrict__ out, int* __restrict__ nrm,
+                                                    int* __restrict__ bad) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row = gid >> 4;
+    const int blk = gid & 15;
+    if (row >= n) return;  // n rows of 16 lanes: whole groups exit together
+    float v[8];
+    load8<TYPE>(in, row, blk, v);
+    bool ok = true;
+    uint32_t w[2] = {0u, 0u};
+    int s = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        ok &= exact_u8(v[i]);
+        const int x = (int)fminf(fmaxf(v[i], 0.f), 255.f);
+        s += x * x;
+        w[i >> 2] |= (uint32_t)x << (8 * (i & 3));
+    }
+    flag_wave(!ok, bad);
+    out[(size_t)row * 16 + blk] = make_uint2(w[0], w[1]);
+    s += __shfl_xor(s, 8, 16);
+    s += __shfl_xor(s, 4, 16);
+    s += __shfl_xor(s, 2, 16);
+    s += __shfl_xor(s, 1, 16);
+    if (blk == 0) nrm[row] = s;
+}
+
 template <int TYPE>
 int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad, int kind,
-                  bool is_base, PackedSet& o, int* bad) {
+                  int role, PackedSet& o, int* bad) {
     const size_t threads = (size_t)rows_pad * 16;
     const unsigned grid = (unsigned)((threads + 255) / 256);
     if (grid == 0) return SIFT_OK;
@@ -226,16 +296,16 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
     auto* nl = static_cast<float*>(o.norm_lane.p);
     switch (kind) {
         case KIND_BF16:
-            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
             break;
         case KIND_F16:
-            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
             break;
         case KIND_F32:
-            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
             break;
         case KIND_I8:
-            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, is_base, bad);
+            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
             break;
         default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
     }
@@ -246,7 +316,8 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
 }  // namespace
 
 int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
-              bool is_base, PackedSet& o, int* bad_dev) {
+              int role, PackedSet& o, int* bad_dev) {
+    const bool is_base = role == PACK_BASE;
     if (rows_pad % TILE_ROWS) return set_error(h, SIFT_ERR_INVALID, "pack: rows_pad not a multiple of 32");
     const uint32_t ntiles = rows_pad / TILE_ROWS;
     SIFT_TRY(reserve(h, o.tiles, (size_t)ntiles * tile_bytes(kind)));
@@ -259,10 +330,10 @@ int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
     ProfScope ps(h, PROF_PACK);
     switch (type) {
         case SIFT_FLOAT_HOST:
-        case SIFT_FLOAT_DEVICE: return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
-        case SIFT_HALF_DEVICE: return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
-        case SIFT_BF16_DEVICE: return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
-        case SIFT_U8_DEVICE: return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, kind, is_base, o, bad_dev);
+        case SIFT_FLOAT_DEVICE: return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
+        case SIFT_HALF_DEVICE: return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
+        case SIFT_BF16_DEVICE: return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
+        case SIFT_U8_DEVICE: return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
         default: return set_error(h, SIFT_ERR_INVALID, "pack: unknown input type " + std::to_string(type));
     }
 }
@@ -278,6 +349,26 @@ int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_
         case SIFT_BF16_DEVICE: check_kernel<SIFT_BF16_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, bad_dev); break;
         case SIFT_U8_DEVICE: return SIFT_OK;  // always exact
         default: return set_error(h, SIFT_ERR_INVALID, "check: unknown input type");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
+          int* bad_dev) {
+    SIFT_TRY(reserve(h, out, (size_t)n * 128));
+    SIFT_TRY(reserve(h, nrm, (size_t)n * 4));
+    if (n == 0) return SIFT_OK;
+    const unsigned grid = (unsigned)(((size_t)n * 16 + 255) / 256);
+    auto* o = static_cast<uint2*>(out.p);
+    auto* r = static_cast<int*>(nrm.p);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE: to_u8_kernel<SIFT_FLOAT_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o, r, bad_dev); break;
+        case SIFT_HALF_DEVICE: to_u8_kernel<SIFT_HALF_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o, r, bad_dev); break;
+        case SIFT_BF16_DEVICE: to_u8_kernel<SIFT_BF16_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o, r, bad_dev); break;
+        case SIFT_U8_DEVICE: to_u8_kernel<SIFT_U8_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o, r, bad_dev); break;
+        default: return set_error(h, SIFT_ERR_INVALID, "to_u8: unknown input type");
     }
     SIFT_HIP_TRY(h, hipGetLastError());
     return SIFT_OK;

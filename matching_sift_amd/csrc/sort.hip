@@ -1,0 +1,171 @@
+// sort.hip — stable LSD radix sort of (uint32 key, uint32 value) pairs and
+// an exclusive scan, for the LSH bucket build (HBM-bound).
+//
+// Replaces the reference's thrust::sort with an indirect comparator
+// (src/lsh_h.cu:587-591) followed by a host run-length pass (:598-620).  Here
+// the whole table set is sorted on the device in ceil(bits/8) passes of:
+//   hist    : per-block 256-bin digit histogram (LDS atomics), bin-major
+//   scan    : exclusive scan of the histogram -> global digit offsets
+//   scatter : stable scatter; ranks among equal digits come from wave-wide
+//             ballots, so elements keep their input order within a digit.
+#include "internal.h"
+
+namespace sift {
+namespace {
+
+constexpr int RS_T = 256;            // threads per block
+constexpr int RS_I = 16;             // keys per thread
+constexpr int RS_CHUNK = RS_T * RS_I;  // keys per block
+constexpr int RS_W = RS_T / 64;      // waves per block
+constexpr int RS_SUB = RS_CHUNK / RS_W;  // keys per wave, walked in order
+
+__global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                int shift, uint32_t* __restrict__ hist,
+                                                uint32_t nblk) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * RS_CHUNK;
+#pragma unroll
+    for (int i = 0; i < RS_I; i++) {
+        const size_t idx = base + (size_t)i * RS_T + threadIdx.x;
+        if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ kin,
+                                                   const uint32_t* __restrict__ vin,
+                                                   uint32_t* __restrict__ kout,
+                                                   uint32_t* __restrict__ vout, uint32_t n,
+                                                   int shift, const uint32_t* __restrict__ offs,
+                                                   uint32_t nblk) {
+    __shared__ uint32_t wh[RS_W][256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < RS_W; k++) wh[k][threadIdx.x] = 0;
+    __syncthreads();
+    const size_t sub = (size_t)blockIdx.x * RS_CHUNK + (size_t)w * RS_SUB;
+    for (int step = 0; step < RS_SUB / 64; step++) {
+        const size_t idx = sub + (size_t)step * 64 + lane;
+        if (idx < n) atomicAdd(&wh[w][(kin[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    {  // per digit: global offset of this block, then prefix over waves
+        const int bin = threadIdx.x;
+        uint32_t run = offs[(size_t)bin * nblk + blockIdx.x];
+#pragma unroll
+        for (int k = 0; k < RS_W; k++) {
+            const uint32_t c = wh[k][bin];
+            wh[k][bin] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int step = 0; step < RS_SUB / 64; step++) {
+        const size_t idx = sub + (size_t)step * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t key = valid ? kin[idx] : 0u;
+        const uint32_t val = valid ? vin[idx] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(valid && set);
+            peers &= set ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        const uint32_t pos = wh[w][d];
+        if (valid) {
+            kout[pos + rank] = key;
+            vout[pos + rank] = val;
+        }
+        // lowest lane of each digit group advances that digit's cursor
+        if (valid && (peers & lt) == 0) wh[w][d] = pos + (uint32_t)__popcll(peers);
+    }
+}
+
+// In-place exclusive scan of 4096-element blocks; block totals to sums[].
+__global__ __launch_bounds__(256) void scan_blocks(uint32_t* __restrict__ data, uint32_t n,
+                                                   uint32_t* __restrict__ sums) {
+    __shared__ uint32_t wsum[4];
+    const size_t base = (size_t)blockIdx.x * 4096 + (size_t)threadIdx.x * 16;
+    uint32_t v[16];
+    uint32_t t = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        v[i] = (base + i < n) ? data[base + i] : 0u;
+        t += v[i];
+    }
+    // inclusive scan of thread totals across the wave
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t x = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int k = 0; k < w; k++) wpre += wsum[k];
+    uint32_t run = wpre + x - t;  // exclusive prefix of this thread
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (base + i < n) data[base + i] = run;
+        run += v[i];
+    }
+    if (threadIdx.x == 255) sums[blockIdx.x] = wpre + x;
+}
+
+__global__ __launch_bounds__(256) void scan_add(uint32_t* __restrict__ data, uint32_t n,
+                                                const uint32_t* __restrict__ sums) {
+    const size_t base = (size_t)blockIdx.x * 4096;
+    const uint32_t add = sums[blockIdx.x];
+    for (int i = threadIdx.x; i < 4096; i += 256)
+        if (base + i < n) data[base + i] += add;
+}
+
+}  // namespace
+
+// Exclusive scan of n uint32 in place (recursive over 4096-element blocks).
+int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth) {
+    if (n == 0) return SIFT_OK;
+    if (depth >= 4) return set_error(h, SIFT_ERR_UNSUPPORTED, "scan too deep");
+    const uint32_t nb = (n + 4095) / 4096;
+    SIFT_TRY(reserve(h, levels[depth], (size_t)nb * 4));
+    auto* sums = static_cast<uint32_t*>(levels[depth].p);
+    scan_blocks<<<nb, 256, 0, h->stream>>>(data, n, sums);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    if (nb > 1) {
+        SIFT_TRY(exclusive_scan(h, sums, nb, levels, depth + 1));
+        scan_add<<<nb, 256, 0, h->stream>>>(data, n, sums);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
+    return SIFT_OK;
+}
+
+// Stable sort of n (key, value) pairs by the low `bits` of key.  Sorted pairs
+// end in (*keys, *vals); the other buffers are scratch of the same size.
+int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk,
+                     uint32_t** tv, uint32_t n, int bits, DevBuf& hist, DevBuf* levels) {
+    if (n == 0) return SIFT_OK;
+    const uint32_t nblk = (n + RS_CHUNK - 1) / RS_CHUNK;
+    SIFT_TRY(reserve(h, hist, (size_t)nblk * 256 * 4));
+    auto* hp = static_cast<uint32_t*>(hist.p);
+    for (int shift = 0; shift < bits; shift += 8) {
+        rs_hist<<<nblk, RS_T, 0, h->stream>>>(*keys, n, shift, hp, nblk);
+        SIFT_HIP_TRY(h, hipGetLastError());
+        SIFT_TRY(exclusive_scan(h, hp, nblk * 256, levels, 0));
+        rs_scatter<<<nblk, RS_T, 0, h->stream>>>(*keys, *vals, *tk, *tv, n, shift, hp, nblk);
+        SIFT_HIP_TRY(h, hipGetLastError());
+        std::swap(*keys, *tk);
+        std::swap(*vals, *tv);
+    }
+    return SIFT_OK;
+}
+
+}  // namespace sift

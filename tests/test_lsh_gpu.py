@@ -1,0 +1,116 @@
+"""GPU LSH (hyperplane projection, radix-sorted buckets, exact re-rank) against
+the oracle's restatement of src/lsh_h.cu: sign codes (:265-317), stable
+bucket order (:587-591), same-bucket candidates re-ranked exactly."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from matching_sift_amd import (FLOAT_DEVICE, FLOAT_HOST, LSH_HYPERPLANE, LSH_HYPERPLANE_PM1,
+                               DeviceArray, LSHIndex, SiftError, datagen)
+from matching_sift_amd import api
+
+pytestmark = pytest.mark.gpu
+FMAX = np.float32(np.finfo(np.float32).max)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_tables(x, H):
+    return np.stack([oracle.lsh_codes(x, H[t]) for t in range(H.shape[0])])
+
+
+def check_index(ix, q, b, ratio=0.8):
+    L, nbits, n = ix.L, ix.nbits, b.shape[0]
+    H = ix.hyperplanes()
+    assert H.shape == (L, nbits, 128) and np.abs(H).max() <= 127 and (H == np.round(H)).all()
+    cr, cq = oracle_tables(b, H), oracle_tables(q, H)
+    np.testing.assert_array_equal(ix.codes(q), cq)
+    np.testing.assert_array_equal(ix.codes(b[:2048]), cr[:, :2048])
+    keys, rows = ix.buckets()
+    for t in range(L):
+        perm = oracle.lsh_sort(cr[t])
+        np.testing.assert_array_equal(rows[t * n:(t + 1) * n], perm)
+        want = cr[t][perm].astype(np.uint64)
+        if nbits < 32:
+            want |= np.uint64(t) << np.uint64(nbits)
+        np.testing.assert_array_equal(keys[t * n:(t + 1) * n], want.astype(np.uint32))
+    idx, dist, nc, mat = ix.query(q, ratio=ratio)
+    oi, od, _ = oracle.lsh_query(q, b, cq, cr)
+    np.testing.assert_array_equal(idx, oi)
+    want_d = np.where(oi == 0xFFFFFFFF, FMAX, od.astype(np.float32))
+    np.testing.assert_array_equal(dist, want_d)
+    want_nc = sum((cr[t][None, :] == cq[t][:, None]).sum(1) for t in range(L))
+    np.testing.assert_array_equal(nc, want_nc.astype(np.uint32))
+    s4 = np.concatenate([want_d, np.where(oi == 0xFFFFFFFF, FMAX, oi.astype(np.float32))], 1)
+    exp_m = oracle.ratio(s4, ratio)
+    exp_m[oi[:, 0] == 0xFFFFFFFF] = 0xFFFFFFFF
+    np.testing.assert_array_equal(mat, exp_m)
+    return idx, nc
+
+
+@pytest.mark.parametrize("nbits,L", [(1, 1), (8, 2), (12, 4), (16, 8), (20, 3), (24, 2), (32, 1)])
+def test_lsh_matches_oracle(handle, nbits, L):
+    say("lsh", nbits, L)
+    b = datagen.base(12_000, seed=31)
+    q = datagen.queries(b, 300, seed=32)
+    ix = LSHIndex(handle, b, nbits, L, seed=7 + nbits)
+    idx, nc = check_index(ix, q, b)
+    ix.close()
+
+
+def test_lsh_pm1_hyperplanes_and_small_bases(handle):
+    for n in (1, 2, 33, 100):
+        say("lsh small", n)
+        b = datagen.base(n, seed=40 + n)
+        q = datagen.queries(b, 50, seed=41)
+        ix = LSHIndex(handle, b, 4, 3, hash_kind=LSH_HYPERPLANE_PM1, seed=3)
+        H = ix.hyperplanes()
+        assert set(np.unique(H)) <= {-1.0, 1.0}
+        check_index(ix, q, b)
+        ix.close()
+
+
+def test_lsh_recall_rises_with_tables(handle):
+    """More tables -> superset of candidates -> recall can only rise."""
+    b = datagen.base(20_000, seed=51)
+    q = datagen.queries(b, 400, seed=52)
+    exact = oracle.knn2_int(q, b, threads=8)[0][:, 0]
+    rec = []
+    for L in (1, 2, 4, 8):
+        ix = LSHIndex(handle, b, 16, L, seed=99)  # same seed: table t identical across L
+        idx = ix.query(q)[0]
+        rec.append(float((idx[:, 0] == exact).mean()))
+        ix.close()
+    say("recall by L", rec)
+    assert all(x <= y for x, y in zip(rec, rec[1:]))
+    assert rec[-1] > rec[0]
+
+
+def test_lsh_gpu_entry_point(handle):
+    b = datagen.base(8_000, seed=61)
+    q = datagen.queries(b, 200, seed=62)
+    qd, bd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(b)
+    m = DeviceArray.empty((200,), np.uint32)
+    api.lsh_gpu(qd, bd, FLOAT_DEVICE, 200, 8000, m, 0.8, handle, 16, 4, LSH_HYPERPLANE, 12)
+    ix = LSHIndex(handle, b, 12, 4, seed=1234)  # the handle's seed
+    np.testing.assert_array_equal(m.numpy(), ix.query(q, ratio=0.8)[3])
+    ix.close()
+
+
+def test_lsh_rejects_non_integer(handle):
+    b = datagen.unit(100, seed=1)
+    with pytest.raises(SiftError) as e:
+        LSHIndex(handle, b, 8, 1)
+    assert e.value.code == -5
+
+
+def test_lsh_rejects_bad_params(handle):
+    b = datagen.base(64, seed=2)
+    for nbits, L in ((0, 1), (33, 1), (8, 0), (31, 4)):
+        with pytest.raises(SiftError):
+            LSHIndex(handle, b, nbits, L)
