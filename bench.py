@@ -47,6 +47,50 @@ def cpu_baseline(q, b, dev_idx, n_sample, threads):
     return n_sample / dt, dt, ok
 
 
+def lsh_sweep(args, ms, q, b, h, qd, bd):
+    """Config 2: random-hyperplane LSH on the 1M x 10k shapes, recall of the
+    exact 2-NN (GPU brute force) against queries/sec, one JSON line per (b, L)."""
+    M, N = q.shape[0], b.shape[0]
+    exact = ms.knn2(h, qd, bd, compute=ms.COMPUTE_BF16)[0]
+    bits = [int(x) for x in args.lsh_bits.split(",")]
+    tables = [int(x) for x in args.lsh_tables.split(",")]
+    outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
+            ms.DeviceArray.empty((M,), np.uint32), None)
+    for nb in bits:
+        for L in tables:
+            h.sync()
+            t0 = time.perf_counter()
+            ix = ms.LSHIndex(h, bd, nb, L, seed=1000 + nb, type_=ms.FLOAT_DEVICE)
+            build_s = time.perf_counter() - t0
+            for _ in range(args.warmup):
+                ix.query(qd, outputs=outs, type_=ms.FLOAT_DEVICE, blocking=False)
+            h.sync()
+            h.profile(True)
+            h.profile_reset()
+            t1 = time.perf_counter()
+            for _ in range(args.steps):
+                ix.query(qd, outputs=outs, type_=ms.FLOAT_DEVICE, blocking=False)
+            h.sync()
+            dt = (time.perf_counter() - t1) / args.steps
+            rr_ms, rr_n = h.profile_read("lsh_query")
+            h.profile(False)
+            idx, nc = outs[0].numpy(), outs[2].numpy()
+            r1 = float((idx[:, 0] == exact[:, 0]).mean())
+            r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
+            cbar = float(nc.mean())
+            rr_s = rr_ms / max(1, rr_n) / 1e3
+            bytes_q = 128 + 4 * L + cbar * (4 + 4 + 128) + 16
+            print(json.dumps({
+                "metric": "LSH 2-NN queries/sec at 1M x 128 base", "workload": "lsh",
+                "nbits": nb, "tables": L, "value": M / dt, "unit": "queries/s",
+                "recall_nn1": r1, "recall_2nn": r2, "mean_candidates": cbar,
+                "build_s": build_s, "rerank_ms": rr_s * 1e3,
+                "roofline": {"bound": "hbm", "achieved": bytes_q * M / rr_s / 1e9, "peak": 8000.0,
+                             "unit": "GB/s", "frac": bytes_q * M / rr_s / 8e12,
+                             "bytes_per_query": bytes_q}}), flush=True)
+            ix.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +103,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
+    ap.add_argument("--workload", default="brute", choices=["brute", "lsh"])
+    ap.add_argument("--lsh-bits", default="8,12,16,20,24")
+    ap.add_argument("--lsh-tables", default="1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
@@ -77,6 +124,9 @@ def main():
     h = ms.Handle(seed=1)
     qd, bd = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b)
     M, N = args.m, args.n
+    if args.workload == "lsh":
+        lsh_sweep(args, ms, q, b, h, qd, bd)
+        return
     outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
             ms.DeviceArray.empty((M, 4), np.float32), ms.DeviceArray.empty((M,), np.uint32))
     L = ms.lib()
@@ -102,6 +152,7 @@ def main():
     fused_ms, fused_n = h.profile_read("fused")
     pack_ms, _ = h.profile_read("pack")
     merge_ms, _ = h.profile_read("merge")
+    seed_ms, _ = h.profile_read("seed")
     h.profile(False)
     idx = outs[0].numpy()
     value = M * args.steps / elapsed
@@ -163,8 +214,8 @@ def main():
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
         "index_qps": index_qps,
-        "breakdown_ms_per_step": {"pack": pack_ms / args.steps, "fused": fused_ms / args.steps,
-                                  "merge": merge_ms / args.steps},
+        "breakdown_ms_per_step": {"pack": pack_ms / args.steps, "seed": seed_ms / args.steps,
+                                  "fused": fused_ms / args.steps, "merge": merge_ms / args.steps},
     }
     print(json.dumps(out), flush=True)
 

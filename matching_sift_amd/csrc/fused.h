@@ -56,24 +56,34 @@ struct KT<KIND_I8> {
 
 // Rank keys are float for the float kinds and int32 for KIND_I8; both are
 // carried as 32-bit patterns.  "none" = +inf / INT32_MAX with index UINT32_MAX.
+// ord(): an int with the same order as the key (atomicMin-able); float keys
+// flip the magnitude bits of negatives.
 template <int KIND>
 struct Key {
     using T = float;
     __device__ static float none() { return __builtin_huge_valf(); }
+    __device__ static int ord(float f) {
+        const int i = __float_as_int(f);
+        return i >= 0 ? i : i ^ 0x7FFFFFFF;
+    }
+    __device__ static float from_ord(int o) { return __int_as_float(o >= 0 ? o : o ^ 0x7FFFFFFF); }
 };
 template <>
 struct Key<KIND_I8> {
     using T = int;
     __device__ static int none() { return 0x7FFFFFFF; }
+    __device__ static int ord(int k) { return k; }
+    __device__ static int from_ord(int o) { return o; }
 };
 
-// Smallest float strictly greater than x (x finite or +inf).
+// Smallest key strictly greater than x (+inf / INT32_MAX stay).
 __device__ inline float next_up(float x) {
     if (x == __builtin_huge_valf()) return x;
     if (x == 0.f) return __uint_as_float(1u);
     const uint32_t u = __float_as_uint(x);
     return __uint_as_float(x > 0.f ? u + 1u : u - 1u);
 }
+__device__ inline int next_up(int x) { return x == 0x7FFFFFFF ? x : x + 1; }
 
 template <typename T>
 __device__ inline bool lex_less(T x, uint32_t ix, T y, uint32_t iy) {
@@ -103,7 +113,8 @@ struct FusedArgs {
     const uint4* b_tiles;   // packed base, [n_tiles][G][64]
     const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32] (I8 only)
     uint4* part;            // [n_slabs][q_stride] {key1, idx1, key2, idx2}
-    const float* thresh;    // per-query admission bound (key bits) or null
+    int* bound;             // per-query inclusive upper bound on the 2nd-best key
+                            // (Key::ord form), shared by all blocks; or null
     uint32_t q_n, q_stride;
     uint32_t r_n;           // real base rows; rows >= r_n in the last tile are masked
     int n_tiles, tiles_per_slab, n_slabs, n_groups;
@@ -115,10 +126,13 @@ struct FusedArgs {
 // shape's group count fits the packed query buffer.
 constexpr uint32_t QPAD = 1024;
 using FusedFn = void (*)(const FusedArgs&, unsigned, hipStream_t);
+using OccFn = int (*)();  // resident blocks per CU of the instantiation
 struct FusedShape {
     int kind, qw, waves, tps;
     FusedFn fn;
+    OccFn blocks_per_cu;
 };
+int device_cu_count();
 // SIFT_SHAPE="qw,waves,tps" picks a listed shape of the kind (for tuning);
 // otherwise the kind's default (first listed).
 const FusedShape* pick_shape(int kind);

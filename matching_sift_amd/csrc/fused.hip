@@ -35,6 +35,63 @@
 namespace sift {
 namespace {
 
+// Global -> LDS copies that bypass VGPRs (gfx950 global_load_lds_*): lane l
+// writes its 16 (or 4) bytes to lds + 16*l (4*l).  Written as inline asm so
+// the compiler does not pair them with a conservative vmcnt(0) before every
+// later LDS read; the kernel waits for them explicitly (wait_vm_lgkm0) before
+// the barrier that hands a stage to the other waves.  M0 carries the LDS
+// address and is restored.  Device pass only: the host pass never runs these.
+__device__ inline void dma16(const void* g, const void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t la = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)lds);
+    uint32_t save;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %2, off\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(save)
+        : "s"(la), "v"(g)
+        : "memory");
+#else
+    (void)g;
+    (void)lds;
+#endif
+}
+__device__ inline void dma4(const void* g, const void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint32_t la = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)lds);
+    uint32_t save;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dword %2, off\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(save)
+        : "s"(la), "v"(g)
+        : "memory");
+#else
+    (void)g;
+    (void)lds;
+#endif
+}
+// s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding: vmcnt[3:0] and [15:14],
+// expcnt[6:4] left at its maximum, lgkmcnt[11:8]).
+template <int N>
+__device__ inline void wait_vm_lgkm0() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4));
+}
+// Work-group barrier without the implicit memory fence: every wait it needs
+// is explicit above it.
+__device__ inline void barrier_only() {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <int KIND, int QW, int WAVES, int TPS>
 __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     using KTk = KT<KIND>;
@@ -44,13 +101,19 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     constexpr bool NORM_IN_K = KIND != KIND_I8;  // float kinds: |b|^2 comes out of the MFMA
     constexpr int UPT = G * 64;                   // 16-byte units per tile
     constexpr int NT = WAVES * 64;
-    constexpr int STAGE = TPS * UPT;
-    constexpr int PER_THREAD = (STAGE + NT - 1) / NT;
     static_assert(TPS * 32 <= NT, "one norm per thread per stage");
 
-    // two LDS stages: stage s+1 is written while stage s is consumed
-    __shared__ __align__(16) uint4 s_tiles[2][STAGE];
-    __shared__ __align__(16) float s_nl[2][NORM_IN_K ? 1 : TPS * 32];
+    // Three LDS stages: stage k is consumed while k+1 and k+2 are in flight.
+    // A stage is CH 1-KB chunks, padded to CHP so every wave issues the same
+    // number of copies (NPW) per full stage; I8 adds one 256-byte norm copy
+    // per wave (into the wave's own slot).
+    constexpr int CH = TPS * G;
+    constexpr int NPW = (CH + WAVES - 1) / WAVES;
+    constexpr int CHP = NPW * WAVES;
+    constexpr int NPW_T = NPW + (NORM_IN_K ? 0 : 1);
+    static_assert(NORM_IN_K || TPS * 32 <= 64, "one 64-lane norm copy per stage");
+    __shared__ __align__(16) uint4 s_tiles[3][CHP * 64];
+    __shared__ __align__(16) float s_nl[3][WAVES][NORM_IN_K ? 1 : 64];
 
     const int b = blockIdx.x;
     const int xcd = b & 7, r = b >> 3;
@@ -70,79 +133,125 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
 #pragma unroll
         for (int g = 0; g < G; g++) qop[qb][g] = a.q_tiles[((size_t)(qblk0 + qb) * G + g) * 64 + lane];
 
-    // Per-lane top-2.  With a seeded bound T (strictly above the query's
-    // 2nd-best key over a base sample, hence above its true 2nd-best), both
-    // slots start at (T, none): only rows with key < T are ever inserted, and
-    // every true top-2 row qualifies.
-    key_t m1[QW], m2[QW];
+    // Per-lane top-2 plus the query's shared bound B: an inclusive upper bound
+    // on its 2nd-best key over the whole base (first from a base sample, then
+    // lowered with atomicMin by any block that finds a better 2nd best).  A
+    // row with key > B cannot be among the two nearest, so the admission test
+    // is key < thr = min(m2, next_up(B)); B is re-read once per stage.  Both
+    // slots start at (next_up(B), none), so every true top-2 row qualifies.
+    key_t m1[QW], m2[QW], thr[QW];
     uint32_t i1[QW], i2[QW];
+    const uint32_t lq = (uint32_t)(lane & 31);
 #pragma unroll
     for (int qb = 0; qb < QW; qb++) {
-        const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + (uint32_t)(lane & 31);
+        const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
         key_t init = Key<KIND>::none();
-        if (a.thresh && q < a.q_n) init = __builtin_bit_cast(key_t, a.thresh[q]);
-        m1[qb] = m2[qb] = init;
+        if (a.bound && q < a.q_n) init = next_up(Key<KIND>::from_ord(a.bound[q]));
+        m1[qb] = m2[qb] = thr[qb] = init;
         i1[qb] = i2[qb] = 0xFFFFFFFFu;
     }
 
-    uint4 pre[PER_THREAD];
-    float pre_nl = 0.f;
-    auto load_stage = [&](int ts) {
-        const int avail = min(TPS, t_end - ts) * UPT;
+    auto dma_stage = [&](int ts, int bsel) -> bool {  // true: full stage (NPW_T copies issued)
+        const int nt = min(TPS, t_end - ts);
+        const bool full = nt == TPS;
         const uint4* src = a.b_tiles + (size_t)ts * UPT;
 #pragma unroll
-        for (int u = 0; u < PER_THREAD; u++) {
-            const int idx = u * NT + tid;
-            pre[u] = (idx < avail) ? src[idx] : make_uint4(0u, 0u, 0u, 0u);
+        for (int i = 0; i < NPW; i++) {
+            const int c = wave + i * WAVES;
+            if (full)  // padding slots reload chunk 0 (never read back)
+                dma16(src + (size_t)(c < CH ? c : 0) * 64 + lane, &s_tiles[bsel][c * 64]);
+            else if (c < nt * G)
+                dma16(src + (size_t)c * 64 + lane, &s_tiles[bsel][c * 64]);
         }
         if constexpr (!NORM_IN_K)
-            if (tid < TPS * 32) pre_nl = (tid < min(TPS, t_end - ts) * 32) ? a.b_nl[(size_t)ts * 32 + tid] : 0.f;
-    };
-    auto store_stage = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < PER_THREAD; u++) {
-            const int idx = u * NT + tid;
-            if (PER_THREAD * NT == STAGE || idx < STAGE) s_tiles[buf][idx] = pre[u];
-        }
-        if constexpr (!NORM_IN_K)
-            if (tid < TPS * 32) s_nl[buf][tid] = pre_nl;
+            dma4(a.b_nl + (size_t)ts * 32 + min(lane, nt * 32 - 1), &s_nl[bsel][wave][0]);
+        return full;
     };
 
-    if (t_begin < t_end) {
-        load_stage(t_begin);
-        store_stage(0);
-    }
-    __syncthreads();
+    const int nst = (t_end - t_begin + TPS - 1) / TPS;  // stages of this slab
+    if (nst > 0) dma_stage(t_begin, 0);
+    bool next_full = nst > 1 ? dma_stage(t_begin + TPS, 1) : false;
+    if (next_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+    barrier_only();
     int buf = 0;
-    for (int ts = t_begin; ts < t_end; ts += TPS) {
-        const bool more = ts + TPS < t_end;
-        if (more) load_stage(ts + TPS);  // in flight while this stage is consumed
+    for (int k = 0; k < nst; k++) {
+        const int ts = t_begin + k * TPS;
         const int ntile = min(TPS, t_end - ts);
+        uint4 afr[TPS][G];
+        int nlr[TPS][16];
+        // (tiles past the slab end read stale LDS: in bounds, never flagged)
 #pragma unroll
         for (int tt = 0; tt < TPS; tt++) {
-            if (tt >= ntile) break;
-            const uint4* tp = &s_tiles[buf][tt * UPT];
+#pragma unroll
+            for (int g = 0; g < G; g++) afr[tt][g] = s_tiles[buf][tt * UPT + g * 64 + lane];
+            if constexpr (!NORM_IN_K) {
+                const float4* nlp = reinterpret_cast<const float4*>(&s_nl[buf][wave][tt * 32 + h * 16]);
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float4 v = nlp[c];
+                    nlr[tt][4 * c] = __float_as_int(v.x); nlr[tt][4 * c + 1] = __float_as_int(v.y);
+                    nlr[tt][4 * c + 2] = __float_as_int(v.z); nlr[tt][4 * c + 3] = __float_as_int(v.w);
+                }
+            }
+        }
+        // stage k+2 goes into the buffer consumed at stage k-1 (released by
+        // the barrier that closed stage k-1)
+        const int b2 = buf == 0 ? 2 : buf - 1;
+        const bool issued = k + 2 < nst;
+        const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
+
+        // keys: float kinds t = |b|^2 - 2 q.b straight from the MFMA (queries
+        // are packed as -2q, the norm group adds |b|^2; every partial sum is an
+        // integer below 2^24 for SIFT data, hence exact); I8: t = |b'|^2 - 2 q'.b'
+        auto tile_keys = [&](int tt, key_t (&tv)[QW][16]) {
             acc_t acc[QW];
 #pragma unroll
             for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
 #pragma unroll
             for (int g = 0; g < G; g++) {
-                const uint4 av = tp[g * 64 + lane];
 #pragma unroll
-                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(av, qop[qb][g], acc[qb]);
+                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][g], qop[qb][g], acc[qb]);
             }
-            const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
-            const uint32_t rowbase = tile_row0 + 4u * h;
-            // keys: float kinds t = |b|^2 - 2 q.b straight from the MFMA (the
-            // queries are packed as -2q, the norm group adds |b|^2; all partial
-            // sums are integers below 2^24 for SIFT data, hence exact);
-            // I8: t = |b'|^2 - 2 q'.b' with the norms from LDS
+#pragma unroll
+            for (int qb = 0; qb < QW; qb++)
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    if constexpr (NORM_IN_K)
+                        tv[qb][j] = acc[qb][j];
+                    else
+                        tv[qb][j] = nlr[tt][j] - 2 * acc[qb][j];
+                }
+        };
+
+        // Common path, branch-free over the whole stage so the compiler can
+        // overlap one tile's key tests with the next tile's MFMAs: per tile a
+        // wave-wide flag "some row may enter some lane's top-2".  The last
+        // tile (pad rows) always takes the slow path, which masks them.
+        bool flag[TPS];
+        bool any_flag = false;
+#pragma unroll
+        for (int tt = 0; tt < TPS; tt++) {
             key_t tv[QW][16];
-            if constexpr (NORM_IN_K) {
+            tile_keys(tt, tv);
+            bool f = (ts + tt + 1) * 32 > (int)a.r_n;
 #pragma unroll
-                for (int qb = 0; qb < QW; qb++)
+            for (int qb = 0; qb < QW; qb++)
 #pragma unroll
-                    for (int j = 0; j < 16; j++) tv[qb][j] = acc[qb][j];
+                for (int c = 0; c < 4; c++)
+                    f |= min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3])) < thr[qb];
+            flag[tt] = (__builtin_amdgcn_ballot_w64(f) != 0) & (tt < ntile);
+            any_flag |= flag[tt];
+        }
+        // Slow path (rare once the bound is tight): recompute the flagged
+        // tiles from the resident fragments and insert, 4 rows at a time.
+        if (any_flag) {
+#pragma unroll
+            for (int tt = 0; tt < TPS; tt++) {
+                if (!flag[tt]) continue;
+                key_t tv[QW][16];
+                tile_keys(tt, tv);
+                const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
+                const uint32_t rowbase = tile_row0 + 4u * h;
                 if (tile_row0 + 32u > a.r_n) {  // last tile: pad rows never rank
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
@@ -151,37 +260,13 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
                         for (int qb = 0; qb < QW; qb++) tv[qb][j] = pad ? Key<KIND>::none() : tv[qb][j];
                     }
                 }
-            } else {
-                const float4* nlp = reinterpret_cast<const float4*>(&s_nl[buf][tt * 32 + h * 16]);
-                int nl[16];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const float4 v = nlp[c];
-                    nl[4 * c] = __float_as_int(v.x); nl[4 * c + 1] = __float_as_int(v.y);
-                    nl[4 * c + 2] = __float_as_int(v.z); nl[4 * c + 3] = __float_as_int(v.w);
-                }
-#pragma unroll
-                for (int qb = 0; qb < QW; qb++)
-#pragma unroll
-                    for (int j = 0; j < 16; j++) tv[qb][j] = nl[j] - 2 * acc[qb][j];
-            }
-            // admission: registers 4c..4c+3 hold 4 consecutive rows (8c + 4h + e);
-            // one wave-wide branch per tile, per-group tests only inside it
-            key_t gm[QW][4];
-            bool any = false;
-#pragma unroll
-            for (int qb = 0; qb < QW; qb++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    gm[qb][c] = min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3]));
-                    any |= gm[qb][c] < m2[qb];
-                }
-            if (any) {  // rare once the bound is tight
-#pragma unroll
-                for (int qb = 0; qb < QW; qb++)
+                for (int qb = 0; qb < QW; qb++) {
+                    const uint32_t old2 = i2[qb];
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
-                        if (gm[qb][c] < m2[qb]) {
+                        const key_t gm = min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3]));
+                        if (gm < thr[qb]) {
 #pragma unroll
                             for (int e = 0; e < 4; e++) {
                                 const key_t v = tv[qb][4 * c + e];
@@ -192,13 +277,22 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
                                 m1[qb] = c1 ? v : m1[qb];
                                 i1[qb] = c1 ? row : i1[qb];
                             }
+                            thr[qb] = min(thr[qb], m2[qb]);
                         }
                     }
+                    // a real 2nd best found here bounds the query everywhere
+                    if (a.bound && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
+                        const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
+                        if (q < a.q_n) atomicMin(&a.bound[q], Key<KIND>::ord(m2[qb]));
+                    }
+                }
             }
         }
-        if (more) store_stage(buf ^ 1);  // the other buffer was released by the last barrier
-        __syncthreads();
-        buf ^= 1;
+        if (k + 1 < nst) {  // stage k+1 complete for every wave; buffer buf free
+            if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+            barrier_only();
+        }
+        buf = buf == 2 ? 0 : buf + 1;
     }
 
     // lanes l and l^32 hold disjoint rows of the same query
@@ -227,21 +321,51 @@ void fused_launch(const FusedArgs& a, unsigned grid, hipStream_t s) {
     fused_top2<KIND, QW, WAVES, TPS><<<grid, WAVES * 64, 0, s>>>(a);
 }
 
+template <int KIND, int QW, int WAVES, int TPS>
+int fused_occupancy() {
+    static int n = -1;
+    if (n < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, fused_top2<KIND, QW, WAVES, TPS>, WAVES * 64, 0) !=
+                hipSuccess ||
+            v < 1)
+            v = 1;
+        n = v;
+    }
+    return n;
+}
+#define SHAPE(K, Q, W, T) {K, Q, W, T, fused_launch<K, Q, W, T>, fused_occupancy<K, Q, W, T>}
+
 const FusedShape kShapes[] = {
     // default per kind comes first
-    {KIND_BF16, 2, 4, 2, fused_launch<KIND_BF16, 2, 4, 2>},
-    {KIND_BF16, 2, 4, 4, fused_launch<KIND_BF16, 2, 4, 4>},
-    {KIND_BF16, 2, 8, 2, fused_launch<KIND_BF16, 2, 8, 2>},
-    {KIND_BF16, 4, 4, 2, fused_launch<KIND_BF16, 4, 4, 2>},
-    {KIND_BF16, 1, 8, 4, fused_launch<KIND_BF16, 1, 8, 4>},
-    {KIND_F16, 2, 4, 2, fused_launch<KIND_F16, 2, 4, 2>},
-    {KIND_F32, 2, 4, 1, fused_launch<KIND_F32, 2, 4, 1>},
-    {KIND_I8, 4, 4, 4, fused_launch<KIND_I8, 4, 4, 4>},
-    {KIND_I8, 2, 4, 4, fused_launch<KIND_I8, 2, 4, 4>},
-    {KIND_I8, 4, 8, 4, fused_launch<KIND_I8, 4, 8, 4>},
+    SHAPE(KIND_BF16, 2, 8, 2),
+    SHAPE(KIND_BF16, 2, 4, 2),
+    SHAPE(KIND_BF16, 2, 4, 4),
+    SHAPE(KIND_BF16, 4, 4, 2),
+    SHAPE(KIND_BF16, 1, 8, 4),
+    SHAPE(KIND_BF16, 1, 4, 2),
+    SHAPE(KIND_BF16, 1, 8, 2),
+    SHAPE(KIND_F16, 2, 8, 2),
+    SHAPE(KIND_F16, 2, 4, 2),
+    SHAPE(KIND_F32, 2, 4, 1),
+    SHAPE(KIND_I8, 2, 4, 2),
+    SHAPE(KIND_I8, 4, 4, 2),
+    SHAPE(KIND_I8, 4, 8, 2),
+    SHAPE(KIND_I8, 2, 8, 2),
 };
 
 }  // namespace
+
+int device_cu_count() {
+    static int n = -1;
+    if (n < 0) {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v < 1) v = 256;
+        n = v;
+    }
+    return n;
+}
 
 const FusedShape* pick_shape(int kind) {
     const FusedShape* first = nullptr;

@@ -27,12 +27,12 @@ struct MergeArgs {
     sift_float4* sorted;
     uint32_t* matches;
     float ratio;
-    float* thresh_out;    // seeding mode: write the admission bound instead
+    int* bound_out;       // seeding mode: write the shared 2nd-best bound instead
 };
 
 // One thread per query: lexicographic merge of the slab partials, then
 // d = key + ||q||^2 and the output formats.  In seeding mode the result is
-// the admission bound: the smallest key strictly above the 2nd-best sample key.
+// the query's shared bound: its 2nd-best key over the sample rows.
 template <int KIND>
 __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
@@ -45,15 +45,8 @@ __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
         merge2(m1, i1, m2, i2, __builtin_bit_cast(key_t, p.x), p.y, __builtin_bit_cast(key_t, p.z), p.w);
     }
     const bool has1 = i1 < a.r_n, has2 = i2 < a.r_n;
-    if (a.thresh_out) {
-        key_t bound = Key<KIND>::none();
-        if (has2) {
-            if constexpr (KIND == KIND_I8)
-                bound = m2 < 0x7FFFFFFF ? m2 + 1 : m2;
-            else
-                bound = next_up(m2);
-        }
-        a.thresh_out[q] = __builtin_bit_cast(float, bound);
+    if (a.bound_out) {  // inclusive bound: the 2nd-best key over the sample
+        a.bound_out[q] = Key<KIND>::ord(has2 ? m2 : Key<KIND>::none());
         return;
     }
     float d1 = FLT_MAX, d2 = FLT_MAX;
@@ -103,30 +96,45 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
     const int n_tiles = (int)bs.ntiles;
     const uint32_t q_stride = (uint32_t)n_groups * qpg;
-    // ~8 resident blocks per CU over 256 CUs; slabs a multiple of 8 (one XCD each)
-    const int target_blocks = 256 * 8;
-    int want = std::max(1, std::min(n_tiles, (target_blocks + n_groups - 1) / n_groups));
-    int tps_slab = std::max(1, (n_tiles + want - 1) / want);
-    tps_slab = ((tps_slab + sh.tps - 1) / sh.tps) * sh.tps;
-    int n_slabs = std::max(1, (n_tiles + tps_slab - 1) / tps_slab);
-    n_slabs = ((n_slabs + 7) / 8) * 8;
+    // Grid = query groups x base slabs (slabs a multiple of 8: one XCD each).
+    // Blocks run in rounds of `cap` resident blocks; pick the slab count whose
+    // block total fills its last round best, preferring 2-8 rounds.
+    const int cap = device_cu_count() * sh.blocks_per_cu();
+    const int max_slabs = std::max(8, std::min(1024, ((n_tiles / sh.tps + 7) / 8) * 8));
+    int n_slabs = 8;
+    double best = 1e30;
+    for (int ns = 8; ns <= max_slabs; ns += 8) {
+        const int tps_s = ((n_tiles + ns - 1) / ns + sh.tps - 1) / sh.tps * sh.tps;
+        const int used = (n_tiles + tps_s - 1) / tps_s;        // non-empty slabs
+        const long items = (long)n_groups * used;
+        const long rounds = (items + cap - 1) / cap;
+        double cost = (double)rounds * tps_s;                  // wall time ~ rounds x slab length
+        if (rounds < 2) cost *= 1.02;                          // keep some dynamic balance
+        cost *= 1.0 + 0.0005 * ns / 8;                         // merge and seeding grow with slabs
+        if (cost < best) {
+            best = cost;
+            n_slabs = ns;
+        }
+    }
+    int tps_slab = ((n_tiles + n_slabs - 1) / n_slabs + sh.tps - 1) / sh.tps * sh.tps;
+    tps_slab = std::max(tps_slab, sh.tps);
 
     const bool seed = r_n >= 2 && n_tiles >= 4 * SEED_TILES;
     const int seed_tiles = SEED_TILES;
     int seed_slab_tiles = ((seed_tiles / 8 + sh.tps - 1) / sh.tps) * sh.tps;
     const int seed_slabs = seed ? (((seed_tiles + seed_slab_tiles - 1) / seed_slab_tiles + 7) / 8) * 8 : 0;
     SIFT_TRY(reserve(h, h->part, (size_t)(n_slabs + seed_slabs) * q_stride * sizeof(uint4)));
-    SIFT_TRY(reserve(h, h->thresh, (size_t)q_stride * sizeof(float)));
+    SIFT_TRY(reserve(h, h->thresh, (size_t)q_stride * sizeof(int)));
     uint4* part = static_cast<uint4*>(h->part.p);
-    float* thresh = static_cast<float*>(h->thresh.p);
+    int* bound = static_cast<int*>(h->thresh.p);
 
-    auto mk_fused = [&](int nt, int per_slab, int slabs, uint4* dst, const float* thr) {
+    auto mk_fused = [&](int nt, int per_slab, int slabs, uint4* dst, int* bnd) {
         FusedArgs fa;
         fa.q_tiles = static_cast<const uint4*>(qs.tiles.p);
         fa.b_tiles = static_cast<const uint4*>(bs.tiles.p);
         fa.b_nl = static_cast<const float*>(bs.norm_lane.p);
         fa.part = dst;
-        fa.thresh = thr;
+        fa.bound = bnd;
         fa.q_n = q_n;
         fa.r_n = r_n;
         fa.q_stride = q_stride;
@@ -149,7 +157,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         ma.sorted = out.sorted;
         ma.matches = out.matches;
         ma.ratio = out.ratio;
-        ma.thresh_out = nullptr;
+        ma.bound_out = nullptr;
         return ma;
     };
     auto launch_merge = [&](const MergeArgs& ma) {
@@ -162,7 +170,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         }
     };
 
-    const float* thr = nullptr;
+    int* bnd = nullptr;
     if (seed) {
         uint4* sp = part + (size_t)n_slabs * q_stride;
         FusedArgs fa = mk_fused(seed_tiles, seed_slab_tiles, seed_slabs, sp, nullptr);
@@ -171,15 +179,15 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
             sh.fn(fa, (unsigned)(seed_slabs * n_groups), h->stream);
             SIFT_HIP_TRY(h, hipGetLastError());
             MergeArgs ma = mk_merge(sp, seed_slabs);
-            ma.thresh_out = thresh;
+            ma.bound_out = bound;
             launch_merge(ma);
             SIFT_HIP_TRY(h, hipGetLastError());
         }
-        thr = thresh;
+        bnd = bound;
     }
     {
         ProfScope ps(h, PROF_FUSED);
-        sh.fn(mk_fused(n_tiles, tps_slab, n_slabs, part, thr), (unsigned)(n_slabs * n_groups), h->stream);
+        sh.fn(mk_fused(n_tiles, tps_slab, n_slabs, part, bnd), (unsigned)(n_slabs * n_groups), h->stream);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     {
