@@ -201,6 +201,54 @@ int sift_lsh_info(sift_lsh_index_t index, uint32_t* r_n, int* nbits, int* L);
 int sift_lsh_get_hyperplanes(sift_lsh_index_t index, float* planes);
 int sift_lsh_get_buckets(sift_lsh_index_t index, uint32_t* keys, uint32_t* rows);
 
+/* ---------------------------------------------------------------------------
+ * Batched image pairs: many (query image, base image) pairs in one persistent
+ * launch.  Per pair the result is what cublas_2nn_sift (include/knn_brute.h:22,
+ * src/knn_brute.cu:24) computes for one image pair in src/test.cu, with true
+ * squared L2 and lowest-index ties: 2-NN indices into the pair's base image,
+ * squared distances, and the ratio test.
+ *
+ * Descriptors are CSR: all images of one side concatenated row-wise in one
+ * device array of `type` (1..4), image i = rows [off[i], off[i+1]).  pairs is
+ * uint32[n_pairs][2] = (query image, base image), host memory.  Outputs are
+ * device arrays over the pairs' queries in pair order (pair p's queries start
+ * at the sum of the query-image sizes of pairs 0..p-1):
+ *   idx uint32[Q][2], dist float[Q][2], matches uint32[Q] (idx1 or UINT32_MAX);
+ * each may be NULL.  An empty base image gives UINT32_MAX / FLT_MAX.
+ * ------------------------------------------------------------------------- */
+typedef struct sift_pairs_plan* sift_pairs_plan_t;
+#define SIFT_PAIRS_GRAPH 1u    /* capture the execute sequence in a HIP graph (replayed while
+                                  the pointers and ratio stay the same) */
+#define SIFT_PAIRS_AUTOTUNE 2u /* on the first execute, time the kind's tile shapes on the
+                                  real data and keep the fastest */
+/* compute: BF16, F16, F32 or I8 (AUTO is not accepted: a plan never reads the data at
+ * build time).  BF16, F16 and I8 need integer descriptors in [0,255] (SIFT data) and
+ * are then exact; F32 takes any data.  The plan owns its device buffers; execute
+ * allocates nothing. */
+int sift_pairs_plan_create(sift_handle_t h, int type, int compute, uint32_t n_qimg,
+                           const uint32_t* q_off, uint32_t n_rimg, const uint32_t* r_off,
+                           uint32_t n_pairs, const uint32_t* pairs, unsigned flags,
+                           sift_pairs_plan_t* plan);
+int sift_pairs_plan_execute(sift_handle_t h, sift_pairs_plan_t plan, const void* q_desc,
+                            const void* r_desc, uint32_t* idx, float* dist, uint32_t* matches,
+                            float ratio, int blocking);
+/* Fix the tile shape (qw query blocks per wave, waves per block, tiles per
+ * stage; one of the kind's listed shapes); drops a captured graph. */
+int sift_pairs_plan_set_shape(sift_pairs_plan_t plan, int qw, int waves, int tps);
+/* out[9]: [0] total queries Q, [1] work items, [2] grid blocks, [3..5] shape
+ * (qw, waves, tps), [6] graph captured (0/1), [7] compute kind, [8] 1 if any
+ * executed input held a value that is not an integer in [0,255] (sticky;
+ * results of BF16/F16/I8 plans are then not exact). */
+int sift_pairs_plan_info(sift_pairs_plan_t plan, int64_t* out9);
+int sift_pairs_plan_destroy(sift_pairs_plan_t plan);
+/* One-shot: plan (AUTO allowed: BF16 for integer data in [0,255], else F32),
+ * execute, synchronize, destroy.  type 0 (host float) is staged.  Explicit
+ * BF16/F16/I8 with non-integer data returns SIFT_ERR_NOT_EXACT. */
+int sift_match_pairs(sift_handle_t h, const void* q_desc, const void* r_desc, int type,
+                     uint32_t n_qimg, const uint32_t* q_off, uint32_t n_rimg,
+                     const uint32_t* r_off, uint32_t n_pairs, const uint32_t* pairs,
+                     int compute, uint32_t* idx, float* dist, uint32_t* matches, float ratio);
+
 #ifdef __cplusplus
 }
 #endif

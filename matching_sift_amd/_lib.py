@@ -61,6 +61,12 @@ SIGNATURES = {
     "sift_lsh_info": (i32, [vp, C.POINTER(u32), C.POINTER(i32), C.POINTER(i32)]),
     "sift_lsh_get_hyperplanes": (i32, [vp, vp]),
     "sift_lsh_get_buckets": (i32, [vp, vp, vp]),
+    "sift_pairs_plan_create": (i32, [vp, i32, i32, u32, vp, u32, vp, u32, vp, C.c_uint, C.POINTER(vp)]),
+    "sift_pairs_plan_execute": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32]),
+    "sift_pairs_plan_set_shape": (i32, [vp, i32, i32, i32]),
+    "sift_pairs_plan_info": (i32, [vp, C.POINTER(C.c_int64)]),
+    "sift_pairs_plan_destroy": (i32, [vp]),
+    "sift_match_pairs": (i32, [vp, vp, vp, i32, u32, vp, u32, vp, u32, vp, i32, vp, vp, vp, f32]),
 }
 
 

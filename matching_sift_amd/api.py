@@ -228,6 +228,90 @@ class LSHIndex:
             pass
 
 
+PAIRS_GRAPH = 1
+PAIRS_AUTOTUNE = 2
+
+
+def _u32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint32))
+
+
+class PairsPlan:
+    """Batched image pairs (sift_pairs_plan_*): CSR query/base descriptor sets,
+    pairs[P,2] = (query image, base image); one persistent launch per execute,
+    optionally captured in a HIP graph and autotuned on first execute."""
+
+    def __init__(self, handle, q_off, r_off, pairs, compute=_lib.COMPUTE_BF16,
+                 type_=_lib.FLOAT_DEVICE, flags=PAIRS_GRAPH | PAIRS_AUTOTUNE):
+        self.handle = handle
+        self.q_off, self.r_off, self.pairs = _u32(q_off), _u32(r_off), _u32(pairs).reshape(-1, 2)
+        pl = C.c_void_p()
+        check(lib().sift_pairs_plan_create(handle.h, type_, compute, len(self.q_off) - 1,
+                                           self.q_off.ctypes.data, len(self.r_off) - 1,
+                                           self.r_off.ctypes.data, self.pairs.shape[0],
+                                           self.pairs.ctypes.data, int(flags), C.byref(pl)),
+              handle.h)
+        self.plan = pl.value
+        self.total = int(self.info()[0])
+
+    def info(self):
+        o = (C.c_int64 * 9)()
+        check(lib().sift_pairs_plan_info(self.plan, o))
+        return list(o)
+
+    def set_shape(self, qw, waves, tps):
+        check(lib().sift_pairs_plan_set_shape(self.plan, qw, waves, tps), self.handle.h)
+
+    def outputs(self):
+        Q = self.total
+        return (DeviceArray.empty((Q, 2), np.uint32), DeviceArray.empty((Q, 2), np.float32),
+                DeviceArray.empty((Q,), np.uint32))
+
+    def execute(self, q, r, outputs=None, ratio=0.8, blocking=True):
+        if outputs is None:
+            outputs = self.outputs()
+        ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
+        check(lib().sift_pairs_plan_execute(self.handle.h, self.plan, q.ptr, r.ptr, ptr(outputs[0]),
+                                            ptr(outputs[1]), ptr(outputs[2]), float(ratio),
+                                            1 if blocking else 0), self.handle.h)
+        if not blocking:
+            return outputs
+        return tuple(o.numpy() if o is not None else None for o in outputs)
+
+    def close(self):
+        if self.plan:
+            lib().sift_pairs_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def match_pairs(handle, q, q_off, r, r_off, pairs, compute=_lib.COMPUTE_AUTO, ratio=0.8,
+                type_=None):
+    """One-shot batched pairs (sift_match_pairs).  Returns numpy (idx[Q,2],
+    dist[Q,2], matches[Q]) over the pairs' queries in pair order."""
+    if type_ == _lib.FLOAT_HOST:  # host float32 rows, staged by the library
+        qh = np.ascontiguousarray(q, np.float32)
+        rh = np.ascontiguousarray(r, np.float32)
+        qp, rp, t = qh.ctypes.data, rh.ctypes.data, type_
+    else:
+        qd, rd = to_device(q), to_device(r)
+        qp, rp, t = qd.ptr, rd.ptr, _input_type(qd, type_)
+    q_off, r_off, pairs = _u32(q_off), _u32(r_off), _u32(pairs).reshape(-1, 2)
+    Q = int(sum(int(q_off[a + 1]) - int(q_off[a]) for a in pairs[:, 0]))
+    outs = (DeviceArray.empty((Q, 2), np.uint32), DeviceArray.empty((Q, 2), np.float32),
+            DeviceArray.empty((Q,), np.uint32))
+    check(lib().sift_match_pairs(handle.h, qp, rp, t, len(q_off) - 1,
+                                 q_off.ctypes.data, len(r_off) - 1, r_off.ctypes.data,
+                                 pairs.shape[0], pairs.ctypes.data, compute, outs[0].ptr,
+                                 outs[1].ptr, outs[2].ptr, float(ratio)), handle.h)
+    return tuple(o.numpy() for o in outs)
+
+
 # ---------------------------------------------------- reference-named ----
 
 def cublas_2nn_sift(q_points, r_points, type_, q_n, r_n, matches, threshold, handle,

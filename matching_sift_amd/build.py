@@ -25,7 +25,7 @@ HIP_FLAGS = [
 ]
 
 # fused.hip: keys are never NaN, so min() needs no NaN canonicalisation
-FILE_FLAGS = {"fused.hip": ["-fno-honor-nans"]}
+FILE_FLAGS = {"fused.hip": ["-fno-honor-nans", "-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def _run(cmd, verbose):

@@ -54,7 +54,7 @@ const char* prof_name(int k) {
 }
 
 ProfScope::ProfScope(sift_context* h_, int k_) : h(h_), k(k_) {
-    if (!h->profile) return;
+    if (!h->profile || k < 0) return;  // k < 0: no timing (e.g. under stream capture)
     if (h->npend == h->cap_pend) prof_flush(h);
     if (h->npend == h->cap_pend) return;
     auto& p = h->pend[h->npend];
@@ -62,7 +62,7 @@ ProfScope::ProfScope(sift_context* h_, int k_) : h(h_), k(k_) {
 }
 
 ProfScope::~ProfScope() {
-    if (!h->profile || h->npend == h->cap_pend) return;
+    if (!h->profile || k < 0 || h->npend == h->cap_pend) return;
     auto& p = h->pend[h->npend];
     hipEventRecord(p.b, h->stream);
     p.k = k;

@@ -108,6 +108,12 @@ __device__ inline void merge2(T& a1, uint32_t& ia1, T& a2, uint32_t& ia2, T b1, 
     ia1 = fi;
 }
 
+// One unit of the persistent pairs kernel (see fused.hip SlabTask).
+struct PairTask {
+    int32_t t_begin, t_end, qblk0;
+    uint32_t r_lim, row_off, q_row0, q_count, out_off;
+};
+
 struct FusedArgs {
     const uint4* q_tiles;   // packed queries, [qblocks][G][64]
     const uint4* b_tiles;   // packed base, [n_tiles][G][64]
@@ -118,6 +124,15 @@ struct FusedArgs {
     uint32_t q_n, q_stride;
     uint32_t r_n;           // real base rows; rows >= r_n in the last tile are masked
     int n_tiles, tiles_per_slab, n_slabs, n_groups;
+    // pairs kernel only
+    const PairTask* tasks;
+    int n_tasks;
+    const float* q_norm;  // packed query norms (int32 bits for KIND_I8)
+    uint32_t* out_idx;
+    float* out_dist;
+    uint32_t* out_matches;
+    float ratio;
+    int neg64;  // -64, passed at run time so the I8 key is one v_mad_i32_i24
 };
 
 
@@ -131,7 +146,11 @@ struct FusedShape {
     int kind, qw, waves, tps;
     FusedFn fn;
     OccFn blocks_per_cu;
+    FusedFn pairs_fn;
+    OccFn pairs_blocks_per_cu;
 };
+// Every listed shape of a kind (for the pairs autotuner).
+int shapes_of_kind(int kind, const FusedShape** out, int max);
 int device_cu_count();
 // SIFT_SHAPE="qw,waves,tps" picks a listed shape of the kind (for tuning);
 // otherwise the kind's default (first listed).

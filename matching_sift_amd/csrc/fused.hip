@@ -92,8 +92,23 @@ __device__ inline void barrier_only() {
     asm volatile("" ::: "memory");
 }
 
-template <int KIND, int QW, int WAVES, int TPS>
-__global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
+// One (query group x base tile range) unit of work: the body of both the
+// grid-mapped kernel (one unit per block) and the persistent pairs kernel
+// (units from a task list).  Query index space: packed query rows; a query is
+// valid when (row - q_row0) < q_count.  Base rows >= r_lim are padding.
+struct SlabTask {
+    int t_begin, t_end;  // base tiles [t_begin, t_end)
+    int qblk0;           // first query block of the work-group
+    uint32_t r_lim;      // base row limit (absolute)
+    uint32_t row_off;    // absolute row of base row 0 (subtracted from output indices)
+    uint32_t q_row0;     // packed query row of query 0
+    uint32_t q_count;    // valid queries
+    uint32_t out_off;    // PAIRS: output slot of query 0
+    int slab;            // !PAIRS: partial-result slab
+};
+
+template <int KIND, int QW, int WAVES, int TPS, bool PAIRS>
+__device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& task) {
     using KTk = KT<KIND>;
     using acc_t = typename KTk::acc_t;
     using key_t = typename Key<KIND>::T;
@@ -115,17 +130,10 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     __shared__ __align__(16) uint4 s_tiles[3][CHP * 64];
     __shared__ __align__(16) float s_nl[3][WAVES][NORM_IN_K ? 1 : 64];
 
-    const int b = blockIdx.x;
-    const int xcd = b & 7, r = b >> 3;
-    const int slab = (r / a.n_groups) * 8 + xcd;
-    const int group = r % a.n_groups;
-    if (slab >= a.n_slabs) return;  // uniform per block
-    const int t_begin = slab * a.tiles_per_slab;
-    const int t_end = min(t_begin + a.tiles_per_slab, a.n_tiles);
-
+    const int t_begin = task.t_begin, t_end = task.t_end;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6, h = lane >> 5;
-    const int qblk0 = (group * WAVES + wave) * QW;
+    const int qblk0 = task.qblk0 + wave * QW;  // this wave's first query block
 
     uint4 qop[QW][G];
 #pragma unroll
@@ -141,14 +149,21 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     // slots start at (next_up(B), none), so every true top-2 row qualifies.
     key_t m1[QW], m2[QW], thr[QW];
     uint32_t i1[QW], i2[QW];
+    // Packed mode (pairs kernel, integer kinds): each lane keeps its top-2 as
+    // int32 "composites" key*32 + slot so one v_min + one v_med3 per row
+    // maintain it with no branch (see the insert below).
+    constexpr bool COMP = PAIRS && KIND != KIND_F32;
+    constexpr int CNONE = 0x7FFFFFE0;  // above every real composite, low bits 0
+    int c1[QW], c2[QW];
     const uint32_t lq = (uint32_t)(lane & 31);
 #pragma unroll
     for (int qb = 0; qb < QW; qb++) {
         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
         key_t init = Key<KIND>::none();
-        if (a.bound && q < a.q_n) init = next_up(Key<KIND>::from_ord(a.bound[q]));
+        if (a.bound && q - task.q_row0 < task.q_count) init = next_up(Key<KIND>::from_ord(a.bound[q]));
         m1[qb] = m2[qb] = thr[qb] = init;
         i1[qb] = i2[qb] = 0xFFFFFFFFu;
+        c1[qb] = c2[qb] = CNONE;
     }
 
     auto dma_stage = [&](int ts, int bsel) -> bool {  // true: full stage (NPW_T copies issued)
@@ -223,6 +238,59 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
                 }
         };
 
+        if constexpr (COMP) {
+            // Every row is inserted, branch-free.  A row's composite is
+            // key*32 + 16 + j (j = accumulator register 0..15, increasing
+            // with the row within the lane's half of the tile); entries kept
+            // from earlier tiles are stored as key*32 (slot bits cleared).
+            // Order by composite = order by (key, row): a new row loses a tie
+            // against a kept one (earlier tile, lower index) and wins or loses
+            // within its tile by j.  Keys are exact integers below 2^25 in
+            // magnitude for SIFT data, so key*32 + 31 fits in int32.
+#pragma unroll
+            for (int tt = 0; tt < TPS; tt++) {
+                if (tt >= ntile) break;  // uniform: slab end
+                acc_t acc[QW];
+#pragma unroll
+                for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
+#pragma unroll
+                for (int g = 0; g < G; g++) {
+#pragma unroll
+                    for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][g], qop[qb][g], acc[qb]);
+                }
+                int w[16];  // I8: |b'|^2 * 32 + 16 + j, so composite = w - 64 q'.b' (one v_mad_i32_i24)
+                if constexpr (!NORM_IN_K) {
+#pragma unroll
+                    for (int j = 0; j < 16; j++) w[j] = (nlr[tt][j] << 5) | (16 + j);
+                }
+                // base pad rows carry PAD_KEY, so they need no mask here: they
+                // lose to every real row and are dropped after the loop
+                const uint32_t rowbase = (uint32_t)(ts + tt) * 32u + 4u * h;
+#pragma unroll
+                for (int qb = 0; qb < QW; qb++) {
+#pragma unroll
+                    for (int j = 0; j < 16; j++) {
+                        int cv;
+                        if constexpr (NORM_IN_K)
+                            cv = ((int)acc[qb][j] << 5) | (16 + j);
+                        else
+                            cv = __mul24((int)acc[qb][j], a.neg64) + w[j];
+                        const int lo = min(c1[qb], c2[qb]), hi = max(c1[qb], c2[qb]);
+                        c2[qb] = max(lo, min(hi, cv));  // v_med3_i32: 2nd smallest of three
+                        c1[qb] = min(c1[qb], cv);
+                    }
+                    // new entries carry slot bits: record their rows, then clear
+                    const bool n1 = (c1[qb] & 16) != 0, n2 = (c2[qb] & 16) != 0;
+                    const uint32_t r1 = rowbase + (uint32_t)(c1[qb] & 3) + ((uint32_t)(c1[qb] & 12) << 1);
+                    const uint32_t r2 = rowbase + (uint32_t)(c2[qb] & 3) + ((uint32_t)(c2[qb] & 12) << 1);
+                    // a kept 2nd entry is the old 1st if the 1st was displaced
+                    i2[qb] = n2 ? r2 : (n1 ? i1[qb] : i2[qb]);
+                    i1[qb] = n1 ? r1 : i1[qb];
+                    c1[qb] &= ~31;
+                    c2[qb] &= ~31;
+                }
+            }
+        } else {
         // Common path, branch-free over the whole stage so the compiler can
         // overlap one tile's key tests with the next tile's MFMAs: per tile a
         // wave-wide flag "some row may enter some lane's top-2".  The last
@@ -233,7 +301,7 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
         for (int tt = 0; tt < TPS; tt++) {
             key_t tv[QW][16];
             tile_keys(tt, tv);
-            bool f = (ts + tt + 1) * 32 > (int)a.r_n;
+            bool f = (uint32_t)(ts + tt + 1) * 32u > task.r_lim;
 #pragma unroll
             for (int qb = 0; qb < QW; qb++)
 #pragma unroll
@@ -252,10 +320,10 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
                 tile_keys(tt, tv);
                 const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
                 const uint32_t rowbase = tile_row0 + 4u * h;
-                if (tile_row0 + 32u > a.r_n) {  // last tile: pad rows never rank
+                if (tile_row0 + 32u > task.r_lim) {  // last tile: pad rows never rank
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
-                        const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= a.r_n;
+                        const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= task.r_lim;
 #pragma unroll
                         for (int qb = 0; qb < QW; qb++) tv[qb][j] = pad ? Key<KIND>::none() : tv[qb][j];
                     }
@@ -283,11 +351,12 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
                     // a real 2nd best found here bounds the query everywhere
                     if (a.bound && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
                         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
-                        if (q < a.q_n) atomicMin(&a.bound[q], Key<KIND>::ord(m2[qb]));
+                        if (q - task.q_row0 < task.q_count) atomicMin(&a.bound[q], Key<KIND>::ord(m2[qb]));
                     }
                 }
             }
         }
+        }  // !COMP
         if (k + 1 < nst) {  // stage k+1 complete for every wave; buffer buf free
             if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
             barrier_only();
@@ -295,6 +364,15 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
         buf = buf == 2 ? 0 : buf + 1;
     }
 
+    if constexpr (COMP) {  // composites back to keys; pad rows (>= r_lim) are absent
+#pragma unroll
+        for (int qb = 0; qb < QW; qb++) {
+            if (i1[qb] >= task.r_lim) i1[qb] = 0xFFFFFFFFu;
+            if (i2[qb] >= task.r_lim) i2[qb] = 0xFFFFFFFFu;
+            m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> 5) : Key<KIND>::none();
+            m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> 5) : Key<KIND>::none();
+        }
+    }
     // lanes l and l^32 hold disjoint rows of the same query
 #pragma unroll
     for (int qb = 0; qb < QW; qb++) {
@@ -304,14 +382,91 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
         const uint32_t oi2 = __shfl_xor(i2[qb], 32);
         merge2(m1[qb], i1[qb], m2[qb], i2[qb], o1, oi1, o2, oi2);
         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + (uint32_t)lane;
-        if (h == 0 && q < a.q_n) {
-            uint4 o;
-            o.x = __builtin_bit_cast(uint32_t, m1[qb]);
-            o.y = i1[qb];
-            o.z = __builtin_bit_cast(uint32_t, m2[qb]);
-            o.w = i2[qb];
-            a.part[(size_t)slab * a.q_stride + q] = o;
+        const uint32_t ql = q - task.q_row0;
+        if (h == 0 && ql < task.q_count) {
+            if constexpr (PAIRS) {
+                // single slab per unit: final outputs (d = key + |q|^2, ratio test)
+                const bool h1 = i1[qb] != 0xFFFFFFFFu, h2 = i2[qb] != 0xFFFFFFFFu;
+                float d1 = 3.402823466e+38f, d2 = 3.402823466e+38f;
+                if constexpr (KIND == KIND_I8) {
+                    const int nq = __float_as_int(a.q_norm[q]);
+                    if (h1) d1 = (float)(m1[qb] + nq);
+                    if (h2) d2 = (float)(m2[qb] + nq);
+                } else {
+                    const float nq = a.q_norm[q];
+                    if (h1) d1 = fmaxf(m1[qb] + nq, 0.f);
+                    if (h2) d2 = fmaxf(m2[qb] + nq, 0.f);
+                }
+                const uint32_t o = task.out_off + ql;
+                const uint32_t j1 = h1 ? i1[qb] - task.row_off : 0xFFFFFFFFu;
+                const uint32_t j2 = h2 ? i2[qb] - task.row_off : 0xFFFFFFFFu;
+                if (a.out_idx) {
+                    a.out_idx[2 * (size_t)o] = j1;
+                    a.out_idx[2 * (size_t)o + 1] = j2;
+                }
+                if (a.out_dist) {
+                    a.out_dist[2 * (size_t)o] = d1;
+                    a.out_dist[2 * (size_t)o + 1] = d2;
+                }
+                if (a.out_matches) {
+                    // IEEE sqrt and division; d1 = d2 = 0 gives NaN -> no match
+                    const float r = sqrtf(d1) / sqrtf(d2);
+                    a.out_matches[o] = (h1 && r < a.ratio) ? j1 : 0xFFFFFFFFu;
+                }
+            } else {
+                uint4 o;
+                o.x = __builtin_bit_cast(uint32_t, m1[qb]);
+                o.y = i1[qb];
+                o.z = __builtin_bit_cast(uint32_t, m2[qb]);
+                o.w = i2[qb];
+                a.part[(size_t)task.slab * a.q_stride + q] = o;
+            }
         }
+    }
+}
+
+template <int KIND, int QW, int WAVES, int TPS>
+__global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
+    // block -> (group, slab), XCD-aware: block b runs on XCD b % 8, and all
+    // groups of one slab are placed on the same XCD to share its L2 copy
+    const int b = blockIdx.x;
+    const int xcd = b & 7, r = b >> 3;
+    const int slab = (r / a.n_groups) * 8 + xcd;
+    const int group = r % a.n_groups;
+    if (slab >= a.n_slabs) return;  // uniform per block
+    SlabTask t;
+    t.t_begin = slab * a.tiles_per_slab;
+    t.t_end = min(t.t_begin + a.tiles_per_slab, a.n_tiles);
+    t.qblk0 = group * WAVES * QW;
+    t.r_lim = a.r_n;
+    t.row_off = 0;
+    t.q_row0 = 0;
+    t.q_count = a.q_n;
+    t.out_off = 0;
+    t.slab = slab;
+    fused_slab<KIND, QW, WAVES, TPS, false>(a, t);
+}
+
+// Persistent kernel over a task list (image pairs): block b takes tasks b,
+// b + grid, ... (the list is sorted by decreasing cost); LDS is reused
+// between tasks behind a barrier.
+template <int KIND, int QW, int WAVES, int TPS>
+__global__ __launch_bounds__(WAVES * 64) void pairs_top2(FusedArgs a) {
+    for (int it = blockIdx.x; it < a.n_tasks; it += gridDim.x) {
+        const PairTask pt = a.tasks[it];
+        SlabTask t;
+        t.t_begin = pt.t_begin;
+        t.t_end = pt.t_end;
+        t.qblk0 = pt.qblk0;
+        t.r_lim = pt.r_lim;
+        t.row_off = pt.row_off;
+        t.q_row0 = pt.q_row0;
+        t.q_count = pt.q_count;
+        t.out_off = pt.out_off;
+        t.slab = 0;
+        fused_slab<KIND, QW, WAVES, TPS, true>(a, t);
+        wait_vm_lgkm0<0>();
+        barrier_only();  // the next task refills the LDS stages
     }
 }
 
@@ -319,6 +474,23 @@ template <int KIND, int QW, int WAVES, int TPS>
 void fused_launch(const FusedArgs& a, unsigned grid, hipStream_t s) {
     static_assert(QPAD % (QW * WAVES * 32) == 0, "group size must divide QPAD");
     fused_top2<KIND, QW, WAVES, TPS><<<grid, WAVES * 64, 0, s>>>(a);
+}
+template <int KIND, int QW, int WAVES, int TPS>
+void pairs_launch(const FusedArgs& a, unsigned grid, hipStream_t s) {
+    pairs_top2<KIND, QW, WAVES, TPS><<<grid, WAVES * 64, 0, s>>>(a);
+}
+template <int KIND, int QW, int WAVES, int TPS>
+int pairs_occupancy() {
+    static int n = -1;
+    if (n < 0) {
+        int v = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, pairs_top2<KIND, QW, WAVES, TPS>, WAVES * 64, 0) !=
+                hipSuccess ||
+            v < 1)
+            v = 1;
+        n = v;
+    }
+    return n;
 }
 
 template <int KIND, int QW, int WAVES, int TPS>
@@ -334,7 +506,8 @@ int fused_occupancy() {
     }
     return n;
 }
-#define SHAPE(K, Q, W, T) {K, Q, W, T, fused_launch<K, Q, W, T>, fused_occupancy<K, Q, W, T>}
+#define SHAPE(K, Q, W, T) \
+    {K, Q, W, T, fused_launch<K, Q, W, T>, fused_occupancy<K, Q, W, T>, pairs_launch<K, Q, W, T>, pairs_occupancy<K, Q, W, T>}
 
 const FusedShape kShapes[] = {
     // default per kind comes first
@@ -350,7 +523,6 @@ const FusedShape kShapes[] = {
     SHAPE(KIND_F32, 2, 4, 1),
     SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 4, 4, 2),
-    SHAPE(KIND_I8, 4, 8, 2),
     SHAPE(KIND_I8, 2, 8, 2),
 };
 
@@ -364,6 +536,13 @@ int device_cu_count() {
         if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v < 1) v = 256;
         n = v;
     }
+    return n;
+}
+
+int shapes_of_kind(int kind, const FusedShape** out, int max) {
+    int n = 0;
+    for (const auto& s : kShapes)
+        if (s.kind == kind && n < max) out[n++] = &s;
     return n;
 }
 

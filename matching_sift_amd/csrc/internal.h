@@ -55,6 +55,12 @@ void prof_flush(sift_context* h);  // accumulate completed event pairs (syncs)
 // squared norm there and query rows the matching multiplier, so the MFMA
 // itself adds |b|^2 to -2 q.b.
 enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4 };
+// Key of a base pad row (zero data, this squared norm).  Real keys of SIFT
+// data t = |b|^2 - 2 q.b are at most 128 * 255^2 = 8,323,200 (float kinds) or
+// 128 * 128^2 + 2 * 128^3 = 6,291,456 (I8, values shifted by -128), so a pad
+// row never outranks a real one.  65504 * 256 is exact in the 3-part BF16
+// and FP16 norm split, and key * 32 + 31 still fits in int32.
+constexpr int PAD_KEY = 65504 * 256;
 constexpr int TILE_ROWS = 32;
 inline int groups_per_tile(int kind) {
     return kind == KIND_F32 ? 17 : (kind == KIND_I8 ? 4 : 9);
@@ -78,6 +84,11 @@ struct PackedSet {
 enum PackRole { PACK_PLAIN = 0, PACK_BASE = 1, PACK_QUERY = 2 };
 int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
               int role, PackedSet& out, int* bad_dev);
+// Launch the pack kernel into an already sized set (o.rows_pad rows; no
+// allocation, so it may run under stream capture).  rowmap (device, optional):
+// packed row -> input row, -1 for a pad row; without it rows >= n are pads.
+int pack_launch(sift_context* h, const void* in, int type, uint32_t n, const int32_t* rowmap, int kind,
+                int role, PackedSet& o, int* bad_dev);
 // uint8 copy [n][128] and int32 squared norms of n input rows (flags non-integers).
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
           int* bad_dev);
@@ -94,6 +105,12 @@ struct Knn2Out {
 // Fused distance + top-2 over packed sets, then merge into the outputs.
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
                 uint32_t r_n, const Knn2Out& out);
+
+bool valid_type(int type);
+// Resolve SIFT_COMPUTE_* to a packed kind.  AUTO: integer data in [0,255] ->
+// BF16 (exact), otherwise `fallback`.  AUTO reads one device flag (syncs).
+int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32_t q_n,
+                 uint32_t r_n, int compute, int fallback, int* kind);
 
 // Host pointer for SIFT_FLOAT_HOST input: stage it into a device buffer.
 int stage_host_input(sift_context* h, const void* host, uint32_t n, DevBuf& buf,

@@ -218,7 +218,7 @@ uint32_t query_rows_pad(int kind, uint32_t q_n) {
 
 using namespace sift;
 
-namespace {
+namespace sift {
 
 bool valid_type(int type) { return type >= SIFT_FLOAT_HOST && type <= SIFT_U8_DEVICE; }
 
@@ -250,6 +250,10 @@ int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32
     *kind = bad ? fallback : KIND_BF16;
     return SIFT_OK;
 }
+
+}  // namespace sift
+
+namespace {
 
 int knn2_impl(sift_context* h, const void* q, const void* r, int type, uint32_t q_n, uint32_t r_n,
               int compute, int fallback, const Knn2Out& out, bool blocking) {

@@ -75,16 +75,19 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
                                                    uint32_t rows_pad, uint4* __restrict__ tiles,
                                                    float* __restrict__ norm,
                                                    float* __restrict__ norm_lane, int role,
-                                                   int* __restrict__ bad) {
+                                                   int* __restrict__ bad,
+                                                   const int32_t* __restrict__ rowmap) {
     const bool is_base = role == PACK_BASE;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t row = gid >> 4;
     const int blk = gid & 15;
     if (row >= rows_pad) return;  // rows_pad is a multiple of 32; whole 16-lane groups exit
-    const bool real = row < n;
+    // rowmap (optional): packed row -> input row, -1 for a pad row
+    const int32_t src = rowmap ? rowmap[row] : (row < n ? (int32_t)row : -1);
+    const bool real = src >= 0;
     float v[8];
     if (real) {
-        load8<TYPE>(in, row, blk, v);
+        load8<TYPE>(in, (size_t)src, blk, v);
         bool ok = true;
 #pragma unroll
         for (int i = 0; i < 8; i++) ok &= exact_u8(v[i]);
@@ -196,26 +199,29 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
         norm[row] = nrm;
         if (is_base) {
             const int hl = (i32 >> 2) & 1, j = 4 * (i32 >> 3) + (i32 & 3);
-            norm_lane[(size_t)t * 32 + hl * 16 + j] = nrm;
+            norm_lane[(size_t)t * 32 + hl * 16 + j] = real ? nrm : __int_as_float(PAD_KEY);
         }
     }
     // Norm group (float kinds): threads 0/1 of the row write lane halves 0/1.
     //   base  : |b|^2 (F32), or |b|^2 / 256 split into three parts (BF16/F16,
     //           exact for integer norms below 2^24)
     //   query : the multiplier (1, or 256 per part) so the MFMA adds |b|^2
-    //   pads and plain rows: zero
+    //   base pad rows: PAD_KEY, above every real key, so a pad row can only
+    //   reach a top-2 when the base has fewer than two rows
+    //   plain rows: zero
     if constexpr (KIND != KIND_I8) {
         if (blk < 2) {
             uint4 u = make_uint4(0u, 0u, 0u, 0u);
-            if (blk == 0 && (role == PACK_QUERY || (role == PACK_BASE && real))) {
+            if (blk == 0 && (role == PACK_QUERY || role == PACK_BASE)) {
+                const float bn = real ? nrm : (float)PAD_KEY;
                 if constexpr (KIND == KIND_F32) {
-                    u.x = __float_as_uint(role == PACK_QUERY ? 1.f : nrm);
+                    u.x = __float_as_uint(role == PACK_QUERY ? 1.f : bn);
                 } else if (role == PACK_QUERY) {
                     const uint32_t m = (KIND == KIND_BF16) ? f2bf(256.f) : f2h(256.f);
                     u.x = m | (m << 16);
                     u.y = m;
                 } else {
-                    float v = nrm * (1.f / 256.f);
+                    float v = bn * (1.f / 256.f);
                     uint32_t p[3];
 #pragma unroll
                     for (int i = 0; i < 3; i++) {
@@ -287,7 +293,7 @@ __global__ __launch_bounds__(256) void to_u8_kernel(const void* __restrict__ in,
 
 template <int TYPE>
 int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad, int kind,
-                  int role, PackedSet& o, int* bad) {
+                  int role, PackedSet& o, int* bad, const int32_t* rowmap) {
     const size_t threads = (size_t)rows_pad * 16;
     const unsigned grid = (unsigned)((threads + 255) / 256);
     if (grid == 0) return SIFT_OK;
@@ -296,16 +302,16 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
     auto* nl = static_cast<float*>(o.norm_lane.p);
     switch (kind) {
         case KIND_BF16:
-            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
+            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
             break;
         case KIND_F16:
-            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
+            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
             break;
         case KIND_F32:
-            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
+            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
             break;
         case KIND_I8:
-            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad);
+            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
             break;
         default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
     }
@@ -328,12 +334,22 @@ int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
     o.ntiles = ntiles;
     o.kind = kind;
     ProfScope ps(h, PROF_PACK);
+    return pack_launch(h, in, type, n, nullptr, kind, role, o, bad_dev);
+}
+
+int pack_launch(sift_context* h, const void* in, int type, uint32_t n, const int32_t* rowmap, int kind,
+                int role, PackedSet& o, int* bad_dev) {
+    const uint32_t rows_pad = o.rows_pad;
     switch (type) {
         case SIFT_FLOAT_HOST:
-        case SIFT_FLOAT_DEVICE: return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
-        case SIFT_HALF_DEVICE: return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
-        case SIFT_BF16_DEVICE: return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
-        case SIFT_U8_DEVICE: return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev);
+        case SIFT_FLOAT_DEVICE:
+            return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev, rowmap);
+        case SIFT_HALF_DEVICE:
+            return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev, rowmap);
+        case SIFT_BF16_DEVICE:
+            return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev, rowmap);
+        case SIFT_U8_DEVICE:
+            return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, kind, role, o, bad_dev, rowmap);
         default: return set_error(h, SIFT_ERR_INVALID, "pack: unknown input type " + std::to_string(type));
     }
 }

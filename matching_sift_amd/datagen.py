@@ -91,5 +91,39 @@ def config1(n=1_000_000, m=10_000):
     return queries(b, m), b
 
 
+def pairs_batch(n_pairs, seed=30000, sizes=None, frac_copy=0.4, noise_sd=8.0):
+    """Batched image pairs: pair p has a base image of sizes[p,1] descriptors
+    (as base()) and a query image of sizes[p,0] descriptors, frac_copy of them
+    noisy copies of the base image's rows.  sizes default to pair_sizes().
+    Returns (q[Qtot,128], q_off[P+1], r[Rtot,128], r_off[P+1], pairs[P,2])
+    with pair p = (query image p, base image p)."""
+    if sizes is None:
+        sizes = pair_sizes(n_pairs, seed=seed)
+    sizes = np.asarray(sizes, np.int64).reshape(n_pairs, 2)
+    q_off = np.zeros(n_pairs + 1, np.uint32)
+    r_off = np.zeros(n_pairs + 1, np.uint32)
+    q_off[1:] = np.cumsum(sizes[:, 0])
+    r_off[1:] = np.cumsum(sizes[:, 1])
+    q = np.empty((int(q_off[-1]), 128), np.float32)
+    r = np.empty((int(r_off[-1]), 128), np.float32)
+    for p in range(n_pairs):
+        nq, nr = int(sizes[p, 0]), int(sizes[p, 1])
+        rb = base(nr, seed=seed + 1 + 2 * p)
+        r[r_off[p]:r_off[p + 1]] = rb
+        if nr > 0:
+            q[q_off[p]:q_off[p + 1]] = queries(rb, nq, seed=seed + 2 + 2 * p, frac_copy=frac_copy,
+                                               noise_sd=noise_sd)
+        else:
+            q[q_off[p]:q_off[p + 1]] = base(nq, seed=seed + 2 + 2 * p)
+    pairs = np.stack([np.arange(n_pairs), np.arange(n_pairs)], 1).astype(np.uint32)
+    return q, q_off, r, r_off, pairs
+
+
+def config3(n_pairs=512):
+    """512 image pairs, keypoints per image LogNormal(median 2000, sigma 0.6)
+    clipped to [200, 8000], 40% of each query image noisy copies."""
+    return pairs_batch(n_pairs)
+
+
 def sha256(arr):
     return hashlib.sha256(np.ascontiguousarray(arr).tobytes()).hexdigest()
