@@ -36,15 +36,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(q, b, dev_idx, n_sample, threads):
+def cpu_baseline(q, b, dev_idx, batch, threads, seconds):
+    """The oracle's host_brute over batches of `batch` queries against the
+    full base until `seconds` have passed (at least one batch)."""
     import oracle  # the CPU leg only: timed baseline and checker
-    qs = q[:n_sample]
+    done, ok = 0, True
     t0 = time.perf_counter()
-    s = oracle.host_brute(qs, b, threads=threads)
+    while done < q.shape[0] and (done == 0 or time.perf_counter() - t0 < seconds):
+        s = oracle.host_brute(q[done:done + batch], b, threads=threads)
+        n = s.shape[0]
+        ok &= bool(np.array_equal(s[:, 2].astype(np.uint32), dev_idx[done:done + n, 0]) and
+                   np.array_equal(s[:, 3].astype(np.uint32), dev_idx[done:done + n, 1]))
+        done += n
     dt = time.perf_counter() - t0
-    ok = bool(np.array_equal(s[:, 2].astype(np.uint32), dev_idx[:n_sample, 0]) and
-              np.array_equal(s[:, 3].astype(np.uint32), dev_idx[:n_sample, 1]))
-    return n_sample / dt, dt, ok
+    return done / dt, dt, ok, done
 
 
 def lsh_sweep(args, ms, q, b, h, qd, bd):
@@ -181,6 +186,21 @@ def pairs_bench(args, ms, h, compute):
     print(json.dumps(out), flush=True)
 
 
+def copy_calibration(args):
+    """A plain device copy of known size (torch's elementwise copy kernel):
+    the reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
+    profiled run into bytes (bytes read = bytes written = 1 GiB per launch)."""
+    import torch
+    n = 1 << 28  # float32 elements = 1 GiB
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(args.steps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "launches": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,11 +209,11 @@ def main():
     ap.add_argument("--compute", default="bf16", choices=list(PEAK))
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--m", type=int, default=10_000)
-    ap.add_argument("--cpu-queries", type=int, default=64)
+    ap.add_argument("--cpu-queries", type=int, default=32, help="CPU baseline batch size")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
-    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs"])
+    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy"])
     ap.add_argument("--pairs", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--lsh-bits", default="8,12,16,20,24")
@@ -210,6 +230,9 @@ def main():
 
     compute = {"bf16": ms.COMPUTE_BF16, "f16": ms.COMPUTE_F16, "f32": ms.COMPUTE_F32,
                "i8": ms.COMPUTE_I8}[args.compute]
+    if args.workload == "copy":
+        copy_calibration(args)
+        return
     if args.workload == "pairs":
         pairs_bench(args, ms, ms.Handle(seed=1), compute)
         return
@@ -281,11 +304,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log("cpu baseline: %d queries x %d base on %d threads" % (args.cpu_queries, N, threads))
-        qps_cpu, dt, ok = cpu_baseline(q, b, idx, args.cpu_queries, threads)
+        log("cpu baseline: batches of %d queries x %d base on %d threads for ~%.0f s"
+            % (args.cpu_queries, N, threads, args.cpu_seconds))
+        qps_cpu, dt, ok, done = cpu_baseline(q, b, idx, args.cpu_queries, threads, args.cpu_seconds)
         cpu = {"value": qps_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d queries x full %d base, host_brute restatement (oracle/), "
-                         "%.1f s; indices equal to GPU: %s" % (args.cpu_queries, N, dt, ok)}
+                         "%.1f s; indices equal to GPU: %s" % (done, N, dt, ok)}
 
     out = {
         "metric": "2-NN queries/sec at 1M x 128 base (exact, bit-exact vs host_brute)",
