@@ -17,7 +17,8 @@ struct KT;
 template <>
 struct KT<KIND_BF16> {
     using acc_t = f32x16;
-    static constexpr int G = 9;  // 8 data groups + the norm group
+    static constexpr int TG = 9;  // groups per packed tile: 8 data groups + the norm group
+    static constexpr int G = 8;   // groups the fused kernel multiplies (K = 128)
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
@@ -26,7 +27,8 @@ struct KT<KIND_BF16> {
 template <>
 struct KT<KIND_F16> {
     using acc_t = f32x16;
-    static constexpr int G = 9;
+    static constexpr int TG = 9;
+    static constexpr int G = 8;
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
@@ -35,7 +37,8 @@ struct KT<KIND_F16> {
 template <>
 struct KT<KIND_F32> {
     using acc_t = f32x16;
-    static constexpr int G = 17;
+    static constexpr int TG = 17;
+    static constexpr int G = 16;
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.x), __uint_as_float(b.x), c, 0, 0, 0);
         c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.y), __uint_as_float(b.y), c, 0, 0, 0);
@@ -47,6 +50,7 @@ struct KT<KIND_F32> {
 template <>
 struct KT<KIND_I8> {
     using acc_t = i32x16;
+    static constexpr int TG = 4;
     static constexpr int G = 4;
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a),
@@ -62,6 +66,7 @@ template <int KIND>
 struct Key {
     using T = float;
     __device__ static float none() { return __builtin_huge_valf(); }
+    __device__ static float lowest() { return -__builtin_huge_valf(); }  // admits nothing
     __device__ static int ord(float f) {
         const int i = __float_as_int(f);
         return i >= 0 ? i : i ^ 0x7FFFFFFF;
@@ -72,6 +77,7 @@ template <>
 struct Key<KIND_I8> {
     using T = int;
     __device__ static int none() { return 0x7FFFFFFF; }
+    __device__ static int lowest() { return (int)0x80000000; }
     __device__ static int ord(int k) { return k; }
     __device__ static int from_ord(int o) { return o; }
 };

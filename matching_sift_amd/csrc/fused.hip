@@ -112,23 +112,22 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     using KTk = KT<KIND>;
     using acc_t = typename KTk::acc_t;
     using key_t = typename Key<KIND>::T;
-    constexpr int G = KTk::G;
-    constexpr bool NORM_IN_K = KIND != KIND_I8;  // float kinds: |b|^2 comes out of the MFMA
-    constexpr int UPT = G * 64;                   // 16-byte units per tile
-    constexpr int NT = WAVES * 64;
-    static_assert(TPS * 32 <= NT, "one norm per thread per stage");
+    constexpr int G = KTk::G;    // groups multiplied per tile (K = 128)
+    constexpr int TG = KTk::TG;  // groups stored per packed tile (float kinds: + norm group)
+    constexpr bool FLOAT_KEY = KIND != KIND_I8;
+    constexpr int UPT = G * 64;  // 16-byte units per tile in LDS
 
     // Three LDS stages: stage k is consumed while k+1 and k+2 are in flight.
     // A stage is CH 1-KB chunks, padded to CHP so every wave issues the same
-    // number of copies (NPW) per full stage; I8 adds one 256-byte norm copy
-    // per wave (into the wave's own slot).
+    // number of copies (NPW) per full stage, plus NLC 256-byte copies of the
+    // stage's row norms (accumulator-lane order) into the wave's own slot.
     constexpr int CH = TPS * G;
     constexpr int NPW = (CH + WAVES - 1) / WAVES;
     constexpr int CHP = NPW * WAVES;
-    constexpr int NPW_T = NPW + (NORM_IN_K ? 0 : 1);
-    static_assert(NORM_IN_K || TPS * 32 <= 64, "one 64-lane norm copy per stage");
+    constexpr int NLC = (TPS * 32 + 63) / 64;
+    constexpr int NPW_T = NPW + NLC;
     __shared__ __align__(16) uint4 s_tiles[3][CHP * 64];
-    __shared__ __align__(16) float s_nl[3][WAVES][NORM_IN_K ? 1 : 64];
+    __shared__ __align__(16) float s_nl[3][WAVES][NLC * 64];
 
     const int t_begin = task.t_begin, t_end = task.t_end;
     const int tid = threadIdx.x;
@@ -139,14 +138,15 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
     for (int qb = 0; qb < QW; qb++)
 #pragma unroll
-        for (int g = 0; g < G; g++) qop[qb][g] = a.q_tiles[((size_t)(qblk0 + qb) * G + g) * 64 + lane];
+        for (int g = 0; g < G; g++) qop[qb][g] = a.q_tiles[((size_t)(qblk0 + qb) * TG + g) * 64 + lane];
 
     // Per-lane top-2 plus the query's shared bound B: an inclusive upper bound
     // on its 2nd-best key over the whole base (first from a base sample, then
-    // lowered with atomicMin by any block that finds a better 2nd best).  A
-    // row with key > B cannot be among the two nearest, so the admission test
-    // is key < thr = min(m2, next_up(B)); B is re-read once per stage.  Both
-    // slots start at (next_up(B), none), so every true top-2 row qualifies.
+    // lowered with atomicMin by any block that finds a better 2nd best; a
+    // block reads B when it starts).  A row with key > B cannot be among the
+    // two nearest, so the admission test is key < thr = min(m2, next_up(B)).
+    // Both slots start at (next_up(B), none), so every true top-2 row
+    // qualifies.
     key_t m1[QW], m2[QW], thr[QW];
     uint32_t i1[QW], i2[QW];
     // Packed mode (pairs kernel, integer kinds): each lane keeps its top-2 as
@@ -159,9 +159,12 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
     for (int qb = 0; qb < QW; qb++) {
         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
+        const bool valid = q - task.q_row0 < task.q_count;
         key_t init = Key<KIND>::none();
-        if (a.bound && q - task.q_row0 < task.q_count) init = next_up(Key<KIND>::from_ord(a.bound[q]));
-        m1[qb] = m2[qb] = thr[qb] = init;
+        if (a.bound && valid) init = next_up(Key<KIND>::from_ord(a.bound[q]));
+        // padding queries admit nothing, so they never flag a tile
+        m1[qb] = m2[qb] = init;
+        thr[qb] = valid ? init : Key<KIND>::lowest();
         i1[qb] = i2[qb] = 0xFFFFFFFFu;
         c1[qb] = c2[qb] = CNONE;
     }
@@ -169,17 +172,20 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     auto dma_stage = [&](int ts, int bsel) -> bool {  // true: full stage (NPW_T copies issued)
         const int nt = min(TPS, t_end - ts);
         const bool full = nt == TPS;
-        const uint4* src = a.b_tiles + (size_t)ts * UPT;
+        // chunk c = tile c / G, group c % G (the packed norm group is skipped)
+        const uint4* src = a.b_tiles + (size_t)ts * TG * 64;
+        auto gsrc = [&](int c) { return src + (size_t)((c / G) * TG + c % G) * 64 + lane; };
 #pragma unroll
         for (int i = 0; i < NPW; i++) {
             const int c = wave + i * WAVES;
             if (full)  // padding slots reload chunk 0 (never read back)
-                dma16(src + (size_t)(c < CH ? c : 0) * 64 + lane, &s_tiles[bsel][c * 64]);
+                dma16(gsrc(c < CH ? c : 0), &s_tiles[bsel][c * 64]);
             else if (c < nt * G)
-                dma16(src + (size_t)c * 64 + lane, &s_tiles[bsel][c * 64]);
+                dma16(gsrc(c), &s_tiles[bsel][c * 64]);
         }
-        if constexpr (!NORM_IN_K)
-            dma4(a.b_nl + (size_t)ts * 32 + min(lane, nt * 32 - 1), &s_nl[bsel][wave][0]);
+#pragma unroll
+        for (int c = 0; c < NLC; c++)
+            dma4(a.b_nl + (size_t)ts * 32 + min(c * 64 + lane, nt * 32 - 1), &s_nl[bsel][wave][c * 64]);
         return full;
     };
 
@@ -199,7 +205,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         for (int tt = 0; tt < TPS; tt++) {
 #pragma unroll
             for (int g = 0; g < G; g++) afr[tt][g] = s_tiles[buf][tt * UPT + g * 64 + lane];
-            if constexpr (!NORM_IN_K) {
+            {
                 const float4* nlp = reinterpret_cast<const float4*>(&s_nl[buf][wave][tt * 32 + h * 16]);
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
@@ -215,23 +221,33 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         const bool issued = k + 2 < nst;
         const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
 
-        // keys: float kinds t = |b|^2 - 2 q.b straight from the MFMA (queries
-        // are packed as -2q, the norm group adds |b|^2; every partial sum is an
-        // integer below 2^24 for SIFT data, hence exact); I8: t = |b'|^2 - 2 q'.b'
-        auto tile_keys = [&](int tt, key_t (&tv)[QW][16]) {
-            acc_t acc[QW];
+        // Float kinds: the accumulator starts at the rows' |b|^2 (accumulator
+        // lane order) and the queries are packed as -2q, so the MFMA chain
+        // ends at t = |b|^2 - 2 q.b (every partial sum is an integer of
+        // magnitude below 2^24 for SIFT data, hence exact).  I8 starts at 0
+        // and t = |b'|^2 - 2 q'.b' afterwards.
+        auto tile_acc = [&](int tt, acc_t (&acc)[QW]) {
+            acc_t init{};
+            if constexpr (FLOAT_KEY) {
 #pragma unroll
-            for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
+                for (int j = 0; j < 16; j++) init[j] = __int_as_float(nlr[tt][j]);
+            }
 #pragma unroll
-            for (int g = 0; g < G; g++) {
+            for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][0], qop[qb][0], init);
+#pragma unroll
+            for (int g = 1; g < G; g++) {
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][g], qop[qb][g], acc[qb]);
             }
+        };
+        auto tile_keys = [&](int tt, key_t (&tv)[QW][16]) {
+            acc_t acc[QW];
+            tile_acc(tt, acc);
 #pragma unroll
             for (int qb = 0; qb < QW; qb++)
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
-                    if constexpr (NORM_IN_K)
+                    if constexpr (FLOAT_KEY)
                         tv[qb][j] = acc[qb][j];
                     else
                         tv[qb][j] = nlr[tt][j] - 2 * acc[qb][j];
@@ -251,15 +267,9 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             for (int tt = 0; tt < TPS; tt++) {
                 if (tt >= ntile) break;  // uniform: slab end
                 acc_t acc[QW];
-#pragma unroll
-                for (int qb = 0; qb < QW; qb++) acc[qb] = acc_t{};
-#pragma unroll
-                for (int g = 0; g < G; g++) {
-#pragma unroll
-                    for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][g], qop[qb][g], acc[qb]);
-                }
+                tile_acc(tt, acc);
                 int w[16];  // I8: |b'|^2 * 32 + 16 + j, so composite = w - 64 q'.b' (one v_mad_i32_i24)
-                if constexpr (!NORM_IN_K) {
+                if constexpr (!FLOAT_KEY) {
 #pragma unroll
                     for (int j = 0; j < 16; j++) w[j] = (nlr[tt][j] << 5) | (16 + j);
                 }
@@ -271,7 +281,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         int cv;
-                        if constexpr (NORM_IN_K)
+                        if constexpr (FLOAT_KEY)
                             cv = ((int)acc[qb][j] << 5) | (16 + j);
                         else
                             cv = __mul24((int)acc[qb][j], a.neg64) + w[j];
@@ -292,52 +302,74 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             }
         } else {
         // Common path, branch-free over the whole stage so the compiler can
-        // overlap one tile's key tests with the next tile's MFMAs: per tile a
-        // wave-wide flag "some row may enter some lane's top-2".  The last
-        // tile (pad rows) always takes the slow path, which masks them.
-        bool flag[TPS];
+        // overlap one tile's key tests with the next tile's MFMAs: per tile
+        // and query block a wave-wide flag "some row may enter some lane's
+        // top-2".  The last tile (pad rows) always takes the slow path, which
+        // masks them.
+        uint32_t fmask[TPS];
         bool any_flag = false;
 #pragma unroll
         for (int tt = 0; tt < TPS; tt++) {
             key_t tv[QW][16];
             tile_keys(tt, tv);
-            bool f = (uint32_t)(ts + tt + 1) * 32u > task.r_lim;
+            const bool last = (uint32_t)(ts + tt + 1) * 32u > task.r_lim;
+            fmask[tt] = 0;
 #pragma unroll
-            for (int qb = 0; qb < QW; qb++)
+            for (int qb = 0; qb < QW; qb++) {
+                bool f = last;
 #pragma unroll
                 for (int c = 0; c < 4; c++)
                     f |= min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3])) < thr[qb];
-            flag[tt] = (__builtin_amdgcn_ballot_w64(f) != 0) & (tt < ntile);
-            any_flag |= flag[tt];
+                fmask[tt] |= (__builtin_amdgcn_ballot_w64(f) != 0 ? 1u : 0u) << qb;
+            }
+            if (tt >= ntile) fmask[tt] = 0;
+            any_flag |= fmask[tt] != 0;
         }
         // Slow path (rare once the bound is tight): recompute the flagged
-        // tiles from the resident fragments and insert, 4 rows at a time.
+        // query blocks of the flagged tiles from the resident fragments and
+        // insert, 4 rows at a time.
         if (any_flag) {
 #pragma unroll
             for (int tt = 0; tt < TPS; tt++) {
-                if (!flag[tt]) continue;
-                key_t tv[QW][16];
-                tile_keys(tt, tv);
-                const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
-                const uint32_t rowbase = tile_row0 + 4u * h;
-                if (tile_row0 + 32u > task.r_lim) {  // last tile: pad rows never rank
-#pragma unroll
-                    for (int j = 0; j < 16; j++) {
-                        const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= task.r_lim;
-#pragma unroll
-                        for (int qb = 0; qb < QW; qb++) tv[qb][j] = pad ? Key<KIND>::none() : tv[qb][j];
-                    }
-                }
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) {
+                    if (!((fmask[tt] >> qb) & 1u)) continue;
+                    key_t tv[16];
+                    {
+                        acc_t acc;
+                        acc_t init{};
+                        if constexpr (FLOAT_KEY) {
+#pragma unroll
+                            for (int j = 0; j < 16; j++) init[j] = __int_as_float(nlr[tt][j]);
+                        }
+                        acc = KTk::mma(afr[tt][0], qop[qb][0], init);
+#pragma unroll
+                        for (int g = 1; g < G; g++) acc = KTk::mma(afr[tt][g], qop[qb][g], acc);
+#pragma unroll
+                        for (int j = 0; j < 16; j++) {
+                            if constexpr (FLOAT_KEY)
+                                tv[j] = acc[j];
+                            else
+                                tv[j] = nlr[tt][j] - 2 * acc[j];
+                        }
+                    }
+                    const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
+                    const uint32_t rowbase = tile_row0 + 4u * h;
+                    if (tile_row0 + 32u > task.r_lim) {  // last tile: pad rows never rank
+#pragma unroll
+                        for (int j = 0; j < 16; j++) {
+                            const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= task.r_lim;
+                            tv[j] = pad ? Key<KIND>::none() : tv[j];
+                        }
+                    }
                     const uint32_t old2 = i2[qb];
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
-                        const key_t gm = min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3]));
+                        const key_t gm = min(min(tv[4 * c], tv[4 * c + 1]), min(tv[4 * c + 2], tv[4 * c + 3]));
                         if (gm < thr[qb]) {
 #pragma unroll
                             for (int e = 0; e < 4; e++) {
-                                const key_t v = tv[qb][4 * c + e];
+                                const key_t v = tv[4 * c + e];
                                 const uint32_t row = rowbase + 8u * c + e;
                                 const bool c1 = v < m1[qb], c2 = v < m2[qb];
                                 m2[qb] = c1 ? m1[qb] : (c2 ? v : m2[qb]);
@@ -511,15 +543,14 @@ int fused_occupancy() {
 
 const FusedShape kShapes[] = {
     // default per kind comes first
-    SHAPE(KIND_BF16, 2, 8, 2),
     SHAPE(KIND_BF16, 2, 4, 2),
+    SHAPE(KIND_BF16, 2, 8, 2),
     SHAPE(KIND_BF16, 2, 4, 4),
     SHAPE(KIND_BF16, 4, 4, 2),
-    SHAPE(KIND_BF16, 1, 8, 4),
     SHAPE(KIND_BF16, 1, 4, 2),
     SHAPE(KIND_BF16, 1, 8, 2),
-    SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F16, 2, 4, 2),
+    SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F32, 2, 4, 1),
     SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 4, 4, 2),

@@ -199,7 +199,8 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
         norm[row] = nrm;
         if (is_base) {
             const int hl = (i32 >> 2) & 1, j = 4 * (i32 >> 3) + (i32 & 3);
-            norm_lane[(size_t)t * 32 + hl * 16 + j] = real ? nrm : __int_as_float(PAD_KEY);
+            const float pad = (KIND == KIND_I8) ? __int_as_float(PAD_KEY) : (float)PAD_KEY;
+            norm_lane[(size_t)t * 32 + hl * 16 + j] = real ? nrm : pad;
         }
     }
     // Norm group (float kinds): threads 0/1 of the row write lane halves 0/1.
