@@ -28,8 +28,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12, "i8": 5.0e15}
-DTYPE = {"bf16": "bf16", "f16": "f16", "f32": "f32", "i8": "i8"}
+PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f32": 157.3e12, "i8": 5.0e15, "fp8": 5.0e15}
+DTYPE = {"bf16": "bf16", "f16": "f16", "f32": "f32", "i8": "i8", "fp8": "fp8 (e4m3) + exact int32 re-rank"}
 
 
 def log(*a):
@@ -135,11 +135,21 @@ def pairs_bench(args, ms, h, compute):
     h.sync()
     k_ms, k_n = h.profile_read("pairs")
     p_ms, _ = h.profile_read("pack")
+    rr_ms, _ = h.profile_read("fp8")
     h.profile(False)
     eager.close()
     k_avg = k_ms / max(1, k_n) / 1e3
     achieved = flops / k_avg / 1e12
     peak = PEAK[args.compute] / 1e12
+    recall = None
+    if compute == ms.COMPUTE_FP8:  # against the exact (I8) plan on the same data
+        ex = ms.PairsPlan(h, q_off, r_off, pairs, compute=ms.COMPUTE_I8, flags=0)
+        ei, _, em = ex.execute(qd, rd)
+        ex.close()
+        recall = {"nn1": float((idx[:, 0] == ei[:, 0]).mean()),
+                  "nn2": float(((idx[:, 0] == ei[:, 0]) & (idx[:, 1] == ei[:, 1])).mean()),
+                  "matches": float((mat == em).mean()),
+                  "against": "exact I8 plan on the same pairs"}
     # parity on the first pairs, and the CPU baseline on a bounded sample
     import oracle
     n_check = min(8, args.pairs)
@@ -180,9 +190,12 @@ def pairs_bench(args, ms, h, compute):
                      "frac": achieved / peak, "traffic": None, "kernel": "pairs_top2",
                      "avg_launch_ms": k_avg * 1e3, "flops_per_launch": flops},
         "cpu_baseline": cpu,
-        "breakdown_ms_per_step": {"pack": p_ms / args.steps, "pairs": k_ms / args.steps},
+        "breakdown_ms_per_step": {"pack": p_ms / args.steps, "pairs": k_ms / args.steps,
+                                  "rerank": rr_ms / args.steps},
         "parity_first_pairs": {"pairs": n_check, "bit_exact": ok},
     }
+    if recall is not None:
+        out["recall_vs_exact"] = recall
     print(json.dumps(out), flush=True)
 
 
@@ -229,7 +242,10 @@ def main():
     from matching_sift_amd import datagen
 
     compute = {"bf16": ms.COMPUTE_BF16, "f16": ms.COMPUTE_F16, "f32": ms.COMPUTE_F32,
-               "i8": ms.COMPUTE_I8}[args.compute]
+               "i8": ms.COMPUTE_I8, "fp8": ms.COMPUTE_FP8}[args.compute]
+    if compute == ms.COMPUTE_FP8 and args.workload != "pairs":
+        print("the FP8 coarse pass is built for the pairs workload (--workload pairs)", file=sys.stderr)
+        sys.exit(2)
     if args.workload == "copy":
         copy_calibration(args)
         return

@@ -58,7 +58,7 @@ enum {
     SIFT_COMPUTE_F16 = 2,  /* FP16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F32 = 3,  /* FP32 MFMA                                      */
     SIFT_COMPUTE_I8 = 4,   /* int8 MFMA on (x-128), int32 accumulate         */
-    SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank (lsh/fp8 API) */
+    SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank (pairs plans) */
 };
 
 /* Hash kinds for the LSH index (lsh_type in the reference, src/lsh_h.cu:716-718). */

@@ -11,6 +11,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x16 __attribute__((ext_vector_type(16)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 template <int KIND>
 struct KT;
@@ -23,6 +24,10 @@ struct KT<KIND_BF16> {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
     }
+    static constexpr int NM = G;  // MFMA steps per tile
+    __device__ static acc_t mma_at(const uint4 (&a)[G], const uint4 (&b)[G], int m, acc_t c) {
+        return mma(a[m], b[m], c);
+    }
 };
 template <>
 struct KT<KIND_F16> {
@@ -32,6 +37,10 @@ struct KT<KIND_F16> {
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    }
+    static constexpr int NM = G;  // MFMA steps per tile
+    __device__ static acc_t mma_at(const uint4 (&a)[G], const uint4 (&b)[G], int m, acc_t c) {
+        return mma(a[m], b[m], c);
     }
 };
 template <>
@@ -46,6 +55,10 @@ struct KT<KIND_F32> {
         c = __builtin_amdgcn_mfma_f32_32x32x2f32(__uint_as_float(a.w), __uint_as_float(b.w), c, 0, 0, 0);
         return c;
     }
+    static constexpr int NM = G;  // MFMA steps per tile
+    __device__ static acc_t mma_at(const uint4 (&a)[G], const uint4 (&b)[G], int m, acc_t c) {
+        return mma(a[m], b[m], c);
+    }
 };
 template <>
 struct KT<KIND_I8> {
@@ -55,6 +68,27 @@ struct KT<KIND_I8> {
     __device__ static acc_t mma(uint4 a, uint4 b, acc_t c) {
         return __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a),
                                                      __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
+    }
+    static constexpr int NM = G;  // MFMA steps per tile
+    __device__ static acc_t mma_at(const uint4 (&a)[G], const uint4 (&b)[G], int m, acc_t c) {
+        return mma(a[m], b[m], c);
+    }
+};
+
+// FP8 (OCP e4m3) with a power-of-two scale per row: groups 0-3 hold the
+// data (two K = 64 steps of 2 groups each), group 4 the row's scale byte in
+// its first byte (e8m0, 127 = 1.0), which the scaled MFMA applies.
+template <>
+struct KT<KIND_FP8> {
+    using acc_t = f32x16;
+    static constexpr int TG = 5;
+    static constexpr int G = 5;
+    static constexpr int NM = 2;
+    __device__ static acc_t mma_at(const uint4 (&a)[G], const uint4 (&b)[G], int m, acc_t c) {
+        const uint4 a0 = a[2 * m], a1 = a[2 * m + 1], b0 = b[2 * m], b1 = b[2 * m + 1];
+        const i32x8 av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+        const i32x8 bv = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, c, 0, 0, 0, (int)a[4].x, 0, (int)b[4].x);
     }
 };
 
@@ -118,6 +152,7 @@ __device__ inline void merge2(T& a1, uint32_t& ia1, T& a2, uint32_t& ia2, T b1, 
 struct PairTask {
     int32_t t_begin, t_end, qblk0;
     uint32_t r_lim, row_off, q_row0, q_count, out_off;
+    int32_t slot;  // FP8 coarse pass: which quarter of the base image (candidate slot)
 };
 
 struct FusedArgs {
@@ -139,6 +174,7 @@ struct FusedArgs {
     uint32_t* out_matches;
     float ratio;
     int neg64;  // -64, passed at run time so the I8 key is one v_mad_i32_i24
+    uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
 };
 
 

@@ -233,11 +233,11 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 for (int j = 0; j < 16; j++) init[j] = __int_as_float(nlr[tt][j]);
             }
 #pragma unroll
-            for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][0], qop[qb][0], init);
+            for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma_at(afr[tt], qop[qb], 0, init);
 #pragma unroll
-            for (int g = 1; g < G; g++) {
+            for (int m = 1; m < KTk::NM; m++) {
 #pragma unroll
-                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma(afr[tt][g], qop[qb][g], acc[qb]);
+                for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma_at(afr[tt], qop[qb], m, acc[qb]);
             }
         };
         auto tile_keys = [&](int tt, key_t (&tv)[QW][16]) {
@@ -342,9 +342,9 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                             for (int j = 0; j < 16; j++) init[j] = __int_as_float(nlr[tt][j]);
                         }
-                        acc = KTk::mma(afr[tt][0], qop[qb][0], init);
+                        acc = KTk::mma_at(afr[tt], qop[qb], 0, init);
 #pragma unroll
-                        for (int g = 1; g < G; g++) acc = KTk::mma(afr[tt][g], qop[qb][g], acc);
+                        for (int m = 1; m < KTk::NM; m++) acc = KTk::mma_at(afr[tt], qop[qb], m, acc);
 #pragma unroll
                         for (int j = 0; j < 16; j++) {
                             if constexpr (FLOAT_KEY)
@@ -404,6 +404,20 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> 5) : Key<KIND>::none();
             m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> 5) : Key<KIND>::none();
         }
+    }
+    if constexpr (PAIRS && KIND == KIND_FP8) {
+        // coarse pass: each lane half's top-2 is a candidate list of its own
+        // (slot 4 * cand_slot + 2 * h); the exact re-rank merges them
+#pragma unroll
+        for (int qb = 0; qb < QW; qb++) {
+            const uint32_t ql = (uint32_t)(qblk0 + qb) * 32u + lq - task.q_row0;
+            if (ql < task.q_count) {
+                const size_t o = ((size_t)task.out_off + ql) * 16 + 4 * (size_t)task.slab + 2 * h;
+                a.out_cand[o] = i1[qb] != 0xFFFFFFFFu ? i1[qb] - task.row_off : 0xFFFFFFFFu;
+                a.out_cand[o + 1] = i2[qb] != 0xFFFFFFFFu ? i2[qb] - task.row_off : 0xFFFFFFFFu;
+            }
+        }
+        return;
     }
     // lanes l and l^32 hold disjoint rows of the same query
 #pragma unroll
@@ -495,7 +509,7 @@ __global__ __launch_bounds__(WAVES * 64) void pairs_top2(FusedArgs a) {
         t.q_row0 = pt.q_row0;
         t.q_count = pt.q_count;
         t.out_off = pt.out_off;
-        t.slab = 0;
+        t.slab = pt.slot;
         fused_slab<KIND, QW, WAVES, TPS, true>(a, t);
         wait_vm_lgkm0<0>();
         barrier_only();  // the next task refills the LDS stages
@@ -552,6 +566,9 @@ const FusedShape kShapes[] = {
     SHAPE(KIND_F16, 2, 4, 2),
     SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F32, 2, 4, 1),
+    SHAPE(KIND_FP8, 2, 4, 2),
+    SHAPE(KIND_FP8, 2, 8, 2),
+    SHAPE(KIND_FP8, 4, 4, 2),
     SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 4, 4, 2),
     SHAPE(KIND_I8, 2, 8, 2),

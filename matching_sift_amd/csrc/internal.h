@@ -54,7 +54,7 @@ void prof_flush(sift_context* h);  // accumulate completed event pairs (syncs)
 // float kinds append one "norm group" (K = 128 + 16): base rows carry their
 // squared norm there and query rows the matching multiplier, so the MFMA
 // itself adds |b|^2 to -2 q.b.
-enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4 };
+enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4, KIND_FP8 = 5 };
 // Key of a base pad row (zero data, this squared norm).  Real keys of SIFT
 // data t = |b|^2 - 2 q.b are at most 128 * 255^2 = 8,323,200 (float kinds) or
 // 128 * 128^2 + 2 * 128^3 = 6,291,456 (I8, values shifted by -128), so a pad
@@ -63,7 +63,8 @@ enum Kind { KIND_BF16 = 1, KIND_F16 = 2, KIND_F32 = 3, KIND_I8 = 4 };
 constexpr int PAD_KEY = 65504 * 256;
 constexpr int TILE_ROWS = 32;
 inline int groups_per_tile(int kind) {
-    return kind == KIND_F32 ? 17 : (kind == KIND_I8 ? 4 : 9);
+    // FP8: 4 groups of e4m3 data + 1 group carrying each row's scale byte
+    return kind == KIND_F32 ? 17 : (kind == KIND_I8 ? 4 : (kind == KIND_FP8 ? 5 : 9));
 }
 inline size_t tile_bytes(int kind) { return (size_t)groups_per_tile(kind) * 64 * 16; }
 

@@ -63,6 +63,33 @@ __device__ inline uint32_t f2h(float f) {
     return b;
 }
 __device__ inline bool exact_u8(float x) { return x >= 0.f && x <= 255.f && x == rintf(x); }
+
+// OCP FP8 e4m3 (bias 7, no infinities, largest finite 448), round to nearest
+// even, saturating at 448; v must be finite.  Subnormals step by 2^-9; a
+// normal value 2^E (1 + m/8) is ((E + 7) << 3) | m, so rounding the scaled
+// mantissa up to 8 carries into the exponent by plain addition.
+__device__ inline uint32_t f32_to_e4m3(float v) {
+    const uint32_t sign = (__float_as_uint(v) >> 24) & 0x80u;
+    const float a = fabsf(v);
+    uint32_t code;
+    if (a < 0.015625f) {  // below 2^-6: subnormal range (the top rounds up to code 8 = 2^-6)
+        code = (uint32_t)rintf(a * 512.f);
+    } else {
+        const int e = (int)((__float_as_uint(a) >> 23) & 0xFF) - 127;  // floor(log2 a)
+        const float mant = __uint_as_float(__float_as_uint(a) & 0x807FFFFFu | 0x3F800000u);  // a / 2^e
+        code = (uint32_t)((e + 7) << 3) + (uint32_t)rintf((mant - 1.f) * 8.f);
+    }
+    return sign | min(code, 0x7Eu);
+}
+// Per-row power-of-two scale: the largest k with max * 2^k <= 448, clamped to
+// [-100, 100] (k = 0 for an all-zero row).
+__device__ inline int fp8_exponent(float mx) {
+    if (!(mx > 0.f)) return 0;
+    const int e = (int)((__float_as_uint(mx) >> 23) & 0xFF) - 127;  // mx = m * 2^e, m in [1, 2)
+    const float m = __uint_as_float(__float_as_uint(mx) & 0x807FFFFFu | 0x3F800000u);
+    const int k = (m <= 1.75f ? 8 : 7) - e;
+    return max(-100, min(100, k));
+}
 // One atomic per wave: the lowest active lane with the condition set reports.
 __device__ inline void flag_wave(bool cond, int* flag) {
     const unsigned long long m = __ballot(cond);
@@ -100,7 +127,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
     // scale); the norm below is computed from q itself.
     const float opscale = (role == PACK_QUERY && KIND != KIND_I8) ? -2.f : 1.f;
     const uint32_t t = row >> 5, i32 = row & 31;
-    constexpr int G = (KIND == KIND_F32) ? 17 : (KIND == KIND_I8 ? 4 : 9);
+    constexpr int G = (KIND == KIND_F32) ? 17 : (KIND == KIND_I8 ? 4 : (KIND == KIND_FP8 ? 5 : 9));
     uint4* tile = tiles + (size_t)t * G * 64;
     float nrm;
     if constexpr (KIND == KIND_BF16 || KIND == KIND_F16) {
@@ -163,6 +190,34 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
 #pragma unroll
         for (int i = 0; i < 8; i++) s = fmaf(v[i], v[i], s);
         nrm = s;
+    } else if constexpr (KIND == KIND_FP8) {
+        // e4m3 of x * 2^k (queries: -x * 2^k, with the factor 2 of -2 q.b in
+        // the scale), k per row so its largest |value| maps to at most 448;
+        // the MFMA applies the scale byte 127 - k (queries 128 - k) itself.
+        // Data group c, lane half h, byte p holds dim 64 (c/2) + 32 h + 16 (c%2) + p.
+        float mx = 0.f, s = 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            mx = fmaxf(mx, fabsf(v[i]));
+            s = fmaf(v[i], v[i], s);
+        }
+        mx = fmaxf(mx, __shfl_xor(mx, 8, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 4, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 2, 16));
+        mx = fmaxf(mx, __shfl_xor(mx, 1, 16));
+        const int k = fp8_exponent(mx);
+        const float sc = __uint_as_float((uint32_t)(127 + k) << 23) * (role == PACK_QUERY ? -1.f : 1.f);
+        uint32_t w[2] = {0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 8; i++) w[i >> 2] |= f32_to_e4m3(v[i] * sc) << (8 * (i & 3));
+        const int m = blk >> 3, h = (blk >> 2) & 1, c = 2 * m + ((blk >> 1) & 1);
+        uint2* dst = reinterpret_cast<uint2*>(tile + c * 64 + i32 + 32 * h) + (blk & 1);
+        *dst = make_uint2(w[0], w[1]);
+        if (blk < 2) {  // scale group: lane halves 0/1 of the row
+            const uint32_t sbyte = (uint32_t)(127 - k + (role == PACK_QUERY ? 1 : 0));
+            tile[4 * 64 + i32 + 32 * blk] = make_uint4(sbyte, 0u, 0u, 0u);
+        }
+        nrm = s;
     } else {  // KIND_I8: values shifted by -128; lane half h holds dims 32g + 16h + 0..15
         int si = 0;
         uint32_t w[2] = {0u, 0u};
@@ -210,7 +265,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
     //   base pad rows: PAD_KEY, above every real key, so a pad row can only
     //   reach a top-2 when the base has fewer than two rows
     //   plain rows: zero
-    if constexpr (KIND != KIND_I8) {
+    if constexpr (KIND == KIND_BF16 || KIND == KIND_F16 || KIND == KIND_F32) {
         if (blk < 2) {
             uint4 u = make_uint4(0u, 0u, 0u, 0u);
             if (blk == 0 && (role == PACK_QUERY || role == PACK_BASE)) {
@@ -313,6 +368,9 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
             break;
         case KIND_I8:
             pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            break;
+        case KIND_FP8:
+            pack_kernel<TYPE, KIND_FP8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
             break;
         default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
     }

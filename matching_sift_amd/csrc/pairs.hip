@@ -40,6 +40,9 @@ struct sift_pairs_plan {
     uint32_t q_rows = 0, r_rows = 0;  // input rows of each side
     PackedSet qs, bs;
     DevBuf qmap, rmap, tasks, flag;
+    // FP8 coarse pass + exact re-rank: candidates, uint8 copies of both sides
+    // with int32 norms, and per output query its input row and base offset
+    DevBuf cand, q_u8, q_n2, r_u8, r_n2, qsrc, boff;
     const FusedShape* shape = nullptr;
     int n_tasks = 0;
     unsigned grid = 0;
@@ -58,6 +61,68 @@ namespace {
 
 inline uint32_t ceil32(uint32_t x) { return (x + 31u) & ~31u; }
 
+// FP8 coarse pass: 4 base quarters x 2 lane halves x top-2 = 16 candidates.
+constexpr int FP8_SLOTS = 4;
+
+// Exact re-rank of the FP8 candidates: 16 lanes per query, one candidate each
+// (uint8 rows, int32 |q - b|^2 = |q|^2 + |b|^2 - 2 q.b with v_dot4_u32_u8),
+// then a butterfly over the 16 lanes keeps the two smallest (d, index) pairs,
+// packed as d * 2^32 + index so one integer order is the lexicographic one.
+__global__ __launch_bounds__(256) void fp8_rerank(const uint32_t* __restrict__ cand,
+                                                  const uint4* __restrict__ q_u8, const int* __restrict__ q_n2,
+                                                  const uint4* __restrict__ r_u8, const int* __restrict__ r_n2,
+                                                  const uint32_t* __restrict__ qsrc,
+                                                  const uint32_t* __restrict__ boff, uint32_t total,
+                                                  uint32_t* __restrict__ idx, float* __restrict__ dist,
+                                                  uint32_t* __restrict__ matches, float ratio) {
+    const uint64_t NONE = ~0ull;
+    const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t o = gid >> 4;
+    const int s = gid & 15;
+    if (o >= total) return;  // whole 16-lane groups exit together
+    const uint32_t c = cand[(size_t)o * 16 + s];
+    uint64_t m1 = NONE, m2 = NONE;
+    if (c != 0xFFFFFFFFu) {
+        const uint32_t qr = qsrc[o];
+        const size_t br = (size_t)boff[o] + c;
+        const uint4* qp = q_u8 + (size_t)qr * 8;
+        const uint4* bp = r_u8 + br * 8;
+        uint32_t dot = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const uint4 a = qp[i], b = bp[i];
+            dot = __builtin_amdgcn_udot4(a.x, b.x, dot, false);
+            dot = __builtin_amdgcn_udot4(a.y, b.y, dot, false);
+            dot = __builtin_amdgcn_udot4(a.z, b.z, dot, false);
+            dot = __builtin_amdgcn_udot4(a.w, b.w, dot, false);
+        }
+        const uint32_t d = (uint32_t)(q_n2[qr] + r_n2[br]) - 2u * dot;
+        m1 = ((uint64_t)d << 32) | c;
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        const uint64_t b1 = __shfl_xor(m1, off, 16), b2 = __shfl_xor(m2, off, 16);
+        const uint64_t n2 = min(max(m1, b1), min(m2, b2));  // 2nd smallest of two sorted pairs
+        m1 = min(m1, b1);
+        m2 = n2;
+    }
+    if (s == 0) {
+        const bool h1 = m1 != NONE, h2 = m2 != NONE;
+        const float d1 = h1 ? (float)(uint32_t)(m1 >> 32) : 3.402823466e+38f;
+        const float d2 = h2 ? (float)(uint32_t)(m2 >> 32) : 3.402823466e+38f;
+        const uint32_t j1 = h1 ? (uint32_t)m1 : 0xFFFFFFFFu, j2 = h2 ? (uint32_t)m2 : 0xFFFFFFFFu;
+        if (idx) {
+            idx[2 * (size_t)o] = j1;
+            idx[2 * (size_t)o + 1] = j2;
+        }
+        if (dist) {
+            dist[2 * (size_t)o] = d1;
+            dist[2 * (size_t)o + 1] = d2;
+        }
+        if (matches) matches[o] = (h1 && sqrtf(d1) / sqrtf(d2) < ratio) ? j1 : 0xFFFFFFFFu;
+    }
+}
+
 void drop_graph(sift_pairs_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
@@ -68,7 +133,8 @@ void drop_graph(sift_pairs_plan* p) {
 void free_plan(sift_pairs_plan* p) {
     drop_graph(p);
     for (DevBuf* b : {&p->qs.tiles, &p->qs.norm, &p->qs.norm_lane, &p->bs.tiles, &p->bs.norm,
-                      &p->bs.norm_lane, &p->qmap, &p->rmap, &p->tasks, &p->flag})
+                      &p->bs.norm_lane, &p->qmap, &p->rmap, &p->tasks, &p->flag, &p->cand, &p->q_u8,
+                      &p->q_n2, &p->r_u8, &p->r_n2, &p->qsrc, &p->boff})
         release(*b);
     delete p;
 }
@@ -83,17 +149,24 @@ int build_tasks(sift_pairs_plan* p) {
         const uint32_t qi = p->pairs[2 * i], ri = p->pairs[2 * i + 1];
         const uint32_t nq = p->q_off[qi + 1] - p->q_off[qi];
         const uint32_t nr = p->r_off[ri + 1] - p->r_off[ri];
+        const int32_t t0 = (int32_t)(p->r_row0[ri] / 32), nt = (int32_t)(ceil32(nr) / 32);
+        // FP8: the base image in FP8_SLOTS quarters, each a candidate slot
+        const int slots = p->kind == KIND_FP8 ? FP8_SLOTS : 1;
+        const int32_t per = (nt + slots - 1) / slots;
         for (uint32_t g = 0; g * qpg < nq; g++) {
-            PairTask w;
-            w.t_begin = (int32_t)(p->r_row0[ri] / 32);
-            w.t_end = w.t_begin + (int32_t)(ceil32(nr) / 32);
-            w.qblk0 = (int32_t)((p->q_row0[qi] + g * qpg) / 32);
-            w.r_lim = p->r_row0[ri] + nr;
-            w.row_off = p->r_row0[ri];
-            w.q_row0 = p->q_row0[qi];
-            w.q_count = nq;
-            w.out_off = p->out_off[i];
-            t.push_back(w);
+            for (int sl = 0; sl < slots; sl++) {
+                PairTask w;
+                w.t_begin = std::min(t0 + sl * per, t0 + nt);
+                w.t_end = std::min(t0 + (sl + 1) * per, t0 + nt);
+                w.qblk0 = (int32_t)((p->q_row0[qi] + g * qpg) / 32);
+                w.r_lim = p->r_row0[ri] + nr;
+                w.row_off = p->r_row0[ri];
+                w.q_row0 = p->q_row0[qi];
+                w.q_count = nq;
+                w.out_off = p->out_off[i];
+                w.slot = sl;
+                t.push_back(w);
+            }
         }
     }
     // longest first, so the round-robin over a persistent grid ends evenly
@@ -131,6 +204,7 @@ FusedArgs make_args(sift_pairs_plan* p, uint32_t* idx, float* dist, uint32_t* ma
     a.out_matches = matches;
     a.ratio = ratio;
     a.neg64 = -64;
+    a.out_cand = static_cast<uint32_t*>(p->cand.p);
     return a;
 }
 
@@ -146,10 +220,27 @@ int run_sequence(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx
         SIFT_TRY(pack_launch(h, r, p->type, p->r_rows, static_cast<const int32_t*>(p->rmap.p), p->kind,
                              PACK_BASE, p->bs, flag));
     }
+    if (p->kind == KIND_FP8) {
+        ProfScope ps(h, prof ? PROF_PACK : -1);
+        SIFT_TRY(to_u8(h, q, p->type, p->q_rows, p->q_u8, p->q_n2, flag));
+        SIFT_TRY(to_u8(h, r, p->type, p->r_rows, p->r_u8, p->r_n2, flag));
+    }
     if (p->n_tasks == 0) return SIFT_OK;
-    ProfScope ps(h, prof ? PROF_PAIRS : -1);
-    p->shape->pairs_fn(make_args(p, idx, dist, matches, ratio), p->grid, h->stream);
-    SIFT_HIP_TRY(h, hipGetLastError());
+    {
+        ProfScope ps(h, prof ? PROF_PAIRS : -1);
+        p->shape->pairs_fn(make_args(p, idx, dist, matches, ratio), p->grid, h->stream);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
+    if (p->kind == KIND_FP8) {
+        ProfScope ps(h, prof ? PROF_FP8 : -1);
+        const unsigned grid = (unsigned)((p->total_q * 16 + 255) / 256);
+        fp8_rerank<<<grid, 256, 0, h->stream>>>(
+            static_cast<const uint32_t*>(p->cand.p), static_cast<const uint4*>(p->q_u8.p),
+            static_cast<const int*>(p->q_n2.p), static_cast<const uint4*>(p->r_u8.p),
+            static_cast<const int*>(p->r_n2.p), static_cast<const uint32_t*>(p->qsrc.p),
+            static_cast<const uint32_t*>(p->boff.p), (uint32_t)p->total_q, idx, dist, matches, ratio);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     return SIFT_OK;
 }
 
@@ -319,6 +410,25 @@ int plan_create(sift_context* h, int type, int kind, uint32_t n_qimg, const uint
         SIFT_TRY(reserve(h, p->bs.norm_lane, (size_t)r_pad * 4));
         SIFT_TRY(reserve(h, p->flag, 16));
         SIFT_HIP_TRY(h, hipMemset(p->flag.p, 0, 16));
+        if (kind == KIND_FP8) {
+            SIFT_TRY(reserve(h, p->cand, (size_t)std::max<uint64_t>(1, p->total_q) * 16 * 4));
+            SIFT_TRY(reserve(h, p->q_u8, (size_t)p->q_rows * 128));
+            SIFT_TRY(reserve(h, p->q_n2, (size_t)p->q_rows * 4));
+            SIFT_TRY(reserve(h, p->r_u8, (size_t)p->r_rows * 128));
+            SIFT_TRY(reserve(h, p->r_n2, (size_t)p->r_rows * 4));
+            std::vector<uint32_t> qs_(std::max<uint64_t>(1, p->total_q)), bo(std::max<uint64_t>(1, p->total_q));
+            for (uint32_t i = 0; i < n_pairs; i++) {
+                const uint32_t qi = p->pairs[2 * i], ri = p->pairs[2 * i + 1];
+                for (uint32_t k = 0; k < p->q_off[qi + 1] - p->q_off[qi]; k++) {
+                    qs_[p->out_off[i] + k] = p->q_off[qi] + k;
+                    bo[p->out_off[i] + k] = p->r_off[ri];
+                }
+            }
+            SIFT_TRY(reserve(h, p->qsrc, qs_.size() * 4));
+            SIFT_TRY(reserve(h, p->boff, bo.size() * 4));
+            SIFT_HIP_TRY(h, hipMemcpy(p->qsrc.p, qs_.data(), qs_.size() * 4, hipMemcpyHostToDevice));
+            SIFT_HIP_TRY(h, hipMemcpy(p->boff.p, bo.data(), bo.size() * 4, hipMemcpyHostToDevice));
+        }
         p->shape = pick_shape(kind);
         if (!p->shape) return set_error(h, SIFT_ERR_INVALID, "no tile shape for the compute kind");
         SIFT_TRY(build_tasks(p));
@@ -337,9 +447,10 @@ int kind_of(sift_context* h, int compute, int* kind) {
         case SIFT_COMPUTE_F16: *kind = KIND_F16; return SIFT_OK;
         case SIFT_COMPUTE_F32: *kind = KIND_F32; return SIFT_OK;
         case SIFT_COMPUTE_I8: *kind = KIND_I8; return SIFT_OK;
+        case SIFT_COMPUTE_FP8: *kind = KIND_FP8; return SIFT_OK;
         default:
             return set_error(h, SIFT_ERR_INVALID,
-                             "a pairs plan needs compute BF16, F16, F32 or I8 (got " + std::to_string(compute) + ")");
+                             "a pairs plan needs compute BF16, F16, F32, I8 or FP8 (got " + std::to_string(compute) + ")");
     }
 }
 

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from matching_sift_amd import (COMPUTE_AUTO, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8,
+from matching_sift_amd import (COMPUTE_AUTO, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8, COMPUTE_FP8,
                                FLOAT_DEVICE, FLOAT_HOST, DeviceArray, PairsPlan, SiftError, datagen,
                                knn2, match_pairs, PAIRS_GRAPH, PAIRS_AUTOTUNE)
 
@@ -214,3 +214,61 @@ def test_pairs_bad_params(handle):
     empty = PairsPlan(handle, q_off, r_off, np.zeros((0, 2), np.uint32), type_=FLOAT_DEVICE)
     assert empty.total == 0
     empty.close()
+
+
+FP8_SHAPES = [(2, 4, 2), (2, 8, 2), (4, 4, 2)]
+
+
+def exact_dist(q, r, qi, ri):
+    d = q[qi].astype(np.int64) - r[ri].astype(np.int64)
+    return (d * d).sum(-1)
+
+
+@pytest.mark.parametrize("ties", [False, True])
+def test_pairs_fp8_exact_domain(handle, ties):
+    """Values in [0, 15] are exact in e4m3 at any power-of-two scale and the
+    dot products stay far below 2^24, so the coarse keys equal the true keys;
+    the two nearest rows are then always among the 16 candidates (each is
+    top-2 of its quarter and lane half) and the re-ranked result is exact."""
+    q, q_off, r, r_off, pairs = ragged_batch(7 if ties else 8, ties=ties)
+    q = np.minimum(q, 15.0).astype(np.float32)
+    r = np.minimum(r, 15.0).astype(np.float32)
+    exp = expected_pairs(q, q_off, r, r_off, pairs)
+    qd, rd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(r)
+    plan = PairsPlan(handle, q_off, r_off, pairs, compute=COMPUTE_FP8, flags=0)
+    for shape in FP8_SHAPES:
+        say("fp8 exact domain", "ties" if ties else "natural", shape)
+        plan.set_shape(*shape)
+        assert_same(plan.execute(qd, rd), exp, "fp8 shape %s" % (shape,))
+    plan.close()
+
+
+def test_pairs_fp8_sift_recall(handle):
+    """SIFT-like pairs: every returned distance is the exact squared L2 of
+    the returned row, and the FP8 coarse pass loses the true nearest row for
+    at most 1% of queries (the loss is reported by the bench)."""
+    sizes = datagen.pair_sizes(24, seed=321)
+    q, q_off, r, r_off, pairs = datagen.pairs_batch(24, seed=33000, sizes=sizes)
+    exp = expected_pairs(q, q_off, r, r_off, pairs)
+    qd, rd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(r)
+    plan = PairsPlan(handle, q_off, r_off, pairs, compute=COMPUTE_FP8)
+    outs = plan.outputs()
+    for rep in range(3):  # autotune + capture, then two replays
+        say("fp8 sift rep", rep)
+        idx, dist, mat = plan.execute(qd, rd, outputs=outs)
+        qrow = np.concatenate([np.arange(q_off[a], q_off[a + 1]) for a, _ in pairs])
+        boff = np.concatenate([np.full(q_off[a + 1] - q_off[a], r_off[b]) for a, b in pairs])
+        for k in range(2):
+            ok = idx[:, k] != UMAX
+            d = exact_dist(q, r, qrow[ok], boff[ok] + idx[ok, k])
+            np.testing.assert_array_equal(dist[ok, k], d.astype(np.float32))
+        assert (dist[:, 0] <= dist[:, 1]).all()
+        assert (idx[:, 0] != idx[:, 1]).all()
+        r1 = (idx[:, 0] == exp[0][:, 0]).mean()
+        say("fp8 recall@1", r1)
+        assert r1 >= 0.99
+        # matches follow the ratio test on the returned (exact) distances
+        want = np.where((idx[:, 0] != UMAX) & (np.sqrt(dist[:, 0]) / np.sqrt(dist[:, 1]) < np.float32(0.8)),
+                        idx[:, 0], UMAX)
+        np.testing.assert_array_equal(mat, want)
+    plan.close()
