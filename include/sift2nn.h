@@ -216,6 +216,58 @@ int sift_lsh_get_buckets(sift_lsh_index_t index, uint32_t* keys, uint32_t* rows)
  *   idx uint32[Q][2], dist float[Q][2], matches uint32[Q] (idx1 or UINT32_MAX);
  * each may be NULL.  An empty base image gives UINT32_MAX / FLT_MAX.
  * ------------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------------
+ * Learned hash (no counterpart in the reference): an MLP 128 -> 256 (ReLU) -> b,
+ * b <= 32, whose output signs are the hash bits (bit k set when z_k < 0, the
+ * hyperplane convention of src/lsh_h.cu:302-309).  BF16 weights on the MFMAs,
+ * FP32 accumulation; trained on the GPU with sign binarisation and a
+ * straight-through gradient on positive pairs (x, exact 1st / 2nd nearest base
+ * row of x, labelled by the brute-force kernel).
+ * Parameters (FP32, SIFT_MLP_NPARAM floats): W1[256][128] (layer 1 multiplies
+ * by W1 / 64, keeping stored weights near unit scale for inputs in [0,255]),
+ * b1[256], W2[32][256] (rows >= b unused), b2[32].
+ * ------------------------------------------------------------------------- */
+typedef struct sift_mlp* sift_mlp_t;
+#define SIFT_MLP_NPARAM (256 * 128 + 256 + 32 * 256 + 32)
+typedef struct {
+    int epochs;      /* passes over the labelled pairs */
+    int batch;       /* pairs per step */
+    float lr;        /* Adam step size */
+    float alpha;     /* weight of the bit-balance term */
+    float beta;      /* weight of the bit-decorrelation term */
+    float gamma;     /* weight of the |z| -> 1 term */
+    float max_ratio; /* positives: (x, 1st NN) where sqrt(d1 / d2) < max_ratio (a distinctive
+                        match); 0 = every (x, 1st NN) and (x, 2nd NN) */
+    uint64_t seed;   /* pair shuffling */
+} sift_mlp_train_opts;
+int sift_mlp_create(sift_handle_t h, int nbits, uint64_t seed, sift_mlp_t* mlp);
+int sift_mlp_destroy(sift_mlp_t mlp);
+int sift_mlp_get_params(sift_mlp_t mlp, float* params);       /* host, SIFT_MLP_NPARAM */
+int sift_mlp_set_params(sift_mlp_t mlp, const float* params);
+/* Codes (uint32[n], device) and optionally the outputs z (float[n][32], device). */
+int sift_mlp_codes(sift_handle_t h, sift_mlp_t mlp, const void* x, int type, uint32_t n,
+                   uint32_t* codes, float* z);
+/* One batch of B pairs (x_i, y_i), device rows of `type` (1..4): the loss
+ * weights {alpha, beta, gamma}, gradients (host, SIFT_MLP_NPARAM), loss (host
+ * float[2]: total, Hamming fraction of the pairs) and z (host float[2B][32]). */
+int sift_mlp_grad(sift_handle_t h, sift_mlp_t mlp, const void* x, const void* y, int type,
+                  uint32_t B, const float* loss_w, float* grads, float* loss, float* z);
+/* Label x (device rows) by its exact 2-NN in base with the brute-force kernel,
+ * then train (the first call also centres every output bit on the training
+ * rows); epoch_loss (host, opts->epochs floats) may be NULL. */
+int sift_mlp_train(sift_handle_t h, sift_mlp_t mlp, const void* x, uint32_t n, const void* base,
+                   uint32_t r_n, int type, const sift_mlp_train_opts* opts, float* epoch_loss);
+/* LSH index whose table t hashes with network mlps[t] (all of the same b);
+ * the networks must outlive the index.  Query it with sift_lsh_query. */
+int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_n,
+                           const sift_mlp_t* mlps, int L, sift_lsh_index_t* index);
+/* sift_lsh_query, then exact brute force (bf: the same base, packed) for the
+ * queries with fewer than min_cand candidates; n_fallback (host) = how many. */
+int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t index, sift_bf_index_t bf,
+                            const void* q_points, int type, uint32_t q_n, uint32_t min_cand,
+                            uint32_t* idx, float* dist, uint32_t* ncand, uint32_t* matches,
+                            float ratio, uint32_t* n_fallback);
+
 typedef struct sift_pairs_plan* sift_pairs_plan_t;
 #define SIFT_PAIRS_GRAPH 1u    /* capture the execute sequence in a HIP graph (replayed while
                                   the pointers and ratio stay the same) */

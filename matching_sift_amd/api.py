@@ -159,9 +159,9 @@ class BruteForceIndex:
         return idx.numpy(), dist.numpy()
 
     def close(self):
-        if self.ix:
+        if self.ix and self.handle.h:
             lib().sift_bf_index_destroy(self.ix)
-            self.ix = None
+        self.ix = None
 
     def __del__(self):
         try:
@@ -182,6 +182,37 @@ class LSHIndex:
         check(lib().sift_lsh_build(handle.h, rd.ptr, _input_type(rd, type_), self.n, self.nbits,
                                    self.L, hash_kind, int(seed), C.byref(ix)), handle.h)
         self.ix = ix.value
+
+    @classmethod
+    def learned(cls, handle, r, mlps, type_=None):
+        """Index whose table t hashes with mlps[t] (HashMLP, kept alive by the index)."""
+        self = cls.__new__(cls)
+        self.handle = handle
+        rd = to_device(r)
+        self._keep = rd
+        self._mlps = list(mlps)
+        self.n, self.nbits, self.L = rd.shape[0], self._mlps[0].nbits, len(self._mlps)
+        arr = (C.c_void_p * self.L)(*[m.m for m in self._mlps])
+        ix = C.c_void_p()
+        check(lib().sift_lsh_build_learned(handle.h, rd.ptr, _input_type(rd, type_), self.n, arr, self.L,
+                                           C.byref(ix)), handle.h)
+        self.ix = ix.value
+        return self
+
+    def query_fallback(self, q, bf_index, min_cand=32, ratio=0.8, type_=None, outputs=None):
+        """Query, then exact brute force for queries with fewer than min_cand
+        candidates.  Returns (idx, dist, ncand, matches, n_fallback)."""
+        qd = to_device(q)
+        M = qd.shape[0]
+        if outputs is None:
+            outputs = (DeviceArray.empty((M, 2), np.uint32), DeviceArray.empty((M, 2), np.float32),
+                       DeviceArray.empty((M,), np.uint32), DeviceArray.empty((M,), np.uint32))
+        idx, dist, nc, mat = outputs
+        nf = C.c_uint32()
+        check(lib().sift_lsh_query_fallback(self.handle.h, self.ix, bf_index.ix, qd.ptr, _input_type(qd, type_), M,
+                                            int(min_cand), idx.ptr, dist.ptr, nc.ptr, mat.ptr, float(ratio),
+                                            C.byref(nf)), self.handle.h)
+        return idx.numpy(), dist.numpy(), nc.numpy(), mat.numpy(), nf.value
 
     def query(self, q, ratio=0.8, type_=None, outputs=None, blocking=True):
         qd = to_device(q)
@@ -217,9 +248,80 @@ class LSHIndex:
         return keys, rows
 
     def close(self):
-        if self.ix:
+        if self.ix and self.handle.h:
             lib().sift_lsh_destroy(self.ix)
-            self.ix = None
+        self.ix = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MLP_NPARAM = 256 * 128 + 256 + 32 * 256 + 32
+
+
+class TrainOpts(C.Structure):
+    """sift_mlp_train_opts."""
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("alpha", C.c_float),
+                ("beta", C.c_float), ("gamma", C.c_float), ("max_ratio", C.c_float), ("seed", C.c_uint64)]
+
+
+class HashMLP:
+    """Learned hash 128 -> 256 (ReLU) -> nbits (sift_mlp_*)."""
+
+    def __init__(self, handle, nbits, seed=0):
+        self.handle, self.nbits = handle, int(nbits)
+        m = C.c_void_p()
+        check(lib().sift_mlp_create(handle.h, self.nbits, int(seed), C.byref(m)), handle.h)
+        self.m = m.value
+
+    def params(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        check(lib().sift_mlp_get_params(self.m, p.ctypes.data), self.handle.h)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == MLP_NPARAM
+        check(lib().sift_mlp_set_params(self.m, p.ctypes.data), self.handle.h)
+
+    def codes(self, x, type_=None, with_z=False):
+        xd = to_device(x)
+        n = xd.shape[0]
+        c = DeviceArray.empty((n,), np.uint32)
+        z = DeviceArray.empty((n, 32), np.float32) if with_z else None
+        check(lib().sift_mlp_codes(self.handle.h, self.m, xd.ptr, _input_type(xd, type_), n, c.ptr,
+                                   z.ptr if z is not None else None), self.handle.h)
+        return (c.numpy(), z.numpy()) if with_z else c.numpy()
+
+    def grad(self, x, y, alpha=1.0, beta=1.0, gamma=0.1, type_=None):
+        """One batch: (loss[2], grads[MLP_NPARAM], z[2B, 32])."""
+        xd, yd = to_device(x), to_device(y)
+        B = xd.shape[0]
+        w = np.array([alpha, beta, gamma], np.float32)
+        g = np.empty(MLP_NPARAM, np.float32)
+        loss = np.empty(2, np.float32)
+        z = np.empty((2 * B, 32), np.float32)
+        check(lib().sift_mlp_grad(self.handle.h, self.m, xd.ptr, yd.ptr, _input_type(xd, type_), B, w.ctypes.data,
+                                  g.ctypes.data, loss.ctypes.data, z.ctypes.data), self.handle.h)
+        return loss, g, z
+
+    def train(self, x, base, epochs=10, batch=4096, lr=1e-3, alpha=1.0, beta=1.0, gamma=0.1, max_ratio=0.8,
+              seed=0, type_=None):
+        xd, bd = to_device(x), to_device(base)
+        o = TrainOpts(int(epochs), int(batch), float(lr), float(alpha), float(beta), float(gamma),
+                      float(max_ratio), int(seed))
+        el = np.empty(int(epochs), np.float32)
+        check(lib().sift_mlp_train(self.handle.h, self.m, xd.ptr, xd.shape[0], bd.ptr, bd.shape[0],
+                                   _input_type(xd, type_), C.byref(o), el.ctypes.data), self.handle.h)
+        return el
+
+    def close(self):
+        if self.m and self.handle.h:  # a closed handle already took the device state with it
+            lib().sift_mlp_destroy(self.m)
+        self.m = None
 
     def __del__(self):
         try:
@@ -279,9 +381,9 @@ class PairsPlan:
         return tuple(o.numpy() if o is not None else None for o in outputs)
 
     def close(self):
-        if self.plan:
+        if self.plan and self.handle.h:
             lib().sift_pairs_plan_destroy(self.plan)
-            self.plan = None
+        self.plan = None
 
     def __del__(self):
         try:

@@ -106,6 +106,8 @@ struct Knn2Out {
 // Fused distance + top-2 over packed sets, then merge into the outputs.
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
                 uint32_t r_n, const Knn2Out& out);
+// Packed query rows for q_n queries (whole launch groups).
+uint32_t query_rows_pad(int kind, uint32_t q_n);
 
 bool valid_type(int type);
 // Resolve SIFT_COMPUTE_* to a packed kind.  AUTO: integer data in [0,255] ->
@@ -120,7 +122,21 @@ size_t input_row_bytes(int type);
 // True for device, managed and pinned (mapped) host memory.
 bool device_accessible(const void* p);
 
+// Learned hash (mlp.hip): forward over packed BF16 plain tiles into codes
+// of table `table` (combine: prefix the table index above the b code bits).
+int mlp_forward(sift_mlp* m, const PackedSet& xs, uint32_t n, float* h_out, float* z_out, uint32_t* codes,
+                uint32_t stride, int table, bool combine);
+int mlp_nbits(const sift_mlp* m);
+
 }  // namespace sift
+
+// Brute-force index (knn2.hip): the base packed once.
+struct sift_bf_index {
+    sift_context* h;
+    sift::PackedSet base;
+    uint32_t r_n;
+    int kind;
+};
 
 struct sift_context {
     int device = 0;

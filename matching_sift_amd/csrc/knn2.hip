@@ -425,13 +425,6 @@ void cublas_2nn_brute_f(const float* q_points, const float* r_points, int q_n, i
 // ------------------------------------------------- brute-force index ----
 }  // extern "C"
 
-struct sift_bf_index {
-    sift_context* h;
-    PackedSet base;
-    uint32_t r_n;
-    int kind;
-};
-
 extern "C" {
 
 int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int compute,

@@ -44,6 +44,7 @@ struct sift_lsh_index {
     uint64_t seed = 0;
     std::vector<float> planes;  // [L][nbits][128]
     PackedSet hp;               // hyperplanes packed as B operands, 32 per table
+    std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller)
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
     DevBuf base_u8, base_nrm;   // gather copy
     DevBuf scratch_k, scratch_v, hist, levels[4];
@@ -55,6 +56,9 @@ namespace {
 struct LshScratch {
     DevBuf q_u8, q_nrm, qcodes, flag;
     PackedSet q_tiles;
+    // exact fallback
+    DevBuf fb_nc, fb_list, fb_count, fb_map, fb_out;
+    PackedSet fb_q;
 };
 
 LshScratch* scratch_of(sift_context* h) {
@@ -103,6 +107,32 @@ __global__ __launch_bounds__(256) void lsh_project(const uint4* __restrict__ x_t
 __global__ void iota_mod(uint32_t* __restrict__ v, uint32_t total, uint32_t n) {
     const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < total) v[i] = (uint32_t)(i % n);
+}
+
+// Queries whose candidate count is below min_cand (order of the list is
+// arbitrary; the host sorts it).
+__global__ void select_low(const uint32_t* __restrict__ nc, uint32_t q_n, uint32_t min_cand,
+                           uint32_t* __restrict__ count, uint32_t* __restrict__ list) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < q_n && nc[q] < min_cand) list[atomicAdd(count, 1u)] = q;
+}
+
+// Exact results of the fallback queries into the caller's outputs.
+__global__ void scatter_exact(const int32_t* __restrict__ map, uint32_t cnt, const uint32_t* __restrict__ oi,
+                              const float* __restrict__ od, const uint32_t* __restrict__ om,
+                              uint32_t* __restrict__ idx, float* __restrict__ dist, uint32_t* __restrict__ matches) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    const size_t q = (size_t)map[i];
+    if (idx) {
+        idx[2 * q] = oi[2 * (size_t)i];
+        idx[2 * q + 1] = oi[2 * (size_t)i + 1];
+    }
+    if (dist) {
+        dist[2 * q] = od[2 * (size_t)i];
+        dist[2 * q + 1] = od[2 * (size_t)i + 1];
+    }
+    if (matches) matches[q] = om[i];
 }
 
 // ----------------------------------------------------------------- query --
@@ -262,11 +292,20 @@ void make_planes(std::vector<float>& p, int L, int nbits, int kind, uint64_t see
 int project(sift_context* h, const PackedSet& xs, uint32_t n, const PackedSet& hp, int L, int b,
             uint32_t* codes, uint32_t stride, bool combine) {
     if (n == 0) return SIFT_OK;
+    if (xs.ntiles == 0) return SIFT_OK;
     const unsigned grid = (xs.ntiles + 3) / 4;
     lsh_project<<<grid, 256, 0, h->stream>>>(static_cast<const uint4*>(xs.tiles.p), n, xs.ntiles,
                                              static_cast<const uint4*>(hp.tiles.p), L, codes, stride,
                                              b, combine ? 1 : 0);
     SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+// Codes of every table: hyperplanes (one launch) or one network per table.
+int hash_codes(sift_context* h, const sift_lsh_index* ix, const PackedSet& xs, uint32_t n, uint32_t* codes,
+               uint32_t stride, bool combine) {
+    if (ix->hash_kind != SIFT_LSH_LEARNED) return project(h, xs, n, ix->hp, ix->L, ix->nbits, codes, stride, combine);
+    for (int t = 0; t < ix->L; t++) SIFT_TRY(mlp_forward(ix->mlps[t], xs, n, nullptr, nullptr, codes, stride, t, combine));
     return SIFT_OK;
 }
 
@@ -286,6 +325,54 @@ int check_flag(sift_context* h, int* flag, const char* what) {
     return SIFT_OK;
 }
 
+// Codes of every base row for all tables, stable radix sort of
+// (table << b | code, row), and the uint8 gather copy of the base.
+int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag) {
+    const uint32_t r_n = ix->n;
+    const int L = ix->L, nbits = ix->nbits;
+    int tbits = 0;
+    while ((1 << tbits) < L) tbits++;
+    int rc = SIFT_OK;
+    do {
+        // gather copy + exactness check
+        if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
+        const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
+        if ((rc = pack_rows(h, r, type, r_n, rows_pad, KIND_BF16, PACK_PLAIN, h->b_set, flag)) != SIFT_OK) break;
+        if ((rc = check_flag(h, flag, "sift_lsh_build")) != SIFT_OK) break;
+        const size_t total = (size_t)r_n * L;
+        for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
+            if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
+        if (rc != SIFT_OK) break;
+        auto* keys = static_cast<uint32_t*>(ix->keys.p);
+        auto* vals = static_cast<uint32_t*>(ix->vals.p);
+        auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
+        auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
+        {
+            ProfScope ps(h, PROF_LSH_PROJ);
+            if ((rc = hash_codes(h, ix, h->b_set, r_n, keys, r_n, true)) != SIFT_OK) break;
+        }
+        if (total) {
+            iota_mod<<<(unsigned)((total + 255) / 256), 256, 0, h->stream>>>(vals, (uint32_t)total, r_n);
+            if (hipGetLastError() != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "iota"); break; }
+        }
+        {
+            ProfScope ps(h, PROF_LSH_SORT);
+            if ((rc = radix_sort_pairs(h, &keys, &vals, &tk, &tv, (uint32_t)total, nbits + tbits, ix->hist,
+                                       ix->levels)) != SIFT_OK)
+                break;
+        }
+        // keep the sorted arrays in ix->keys / ix->vals
+        if (keys != ix->keys.p) {
+            std::swap(ix->keys, ix->scratch_k);
+            std::swap(ix->vals, ix->scratch_v);
+        }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "sync"); break; }
+        release(ix->scratch_k);
+        release(ix->scratch_v);
+    } while (0);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -294,7 +381,8 @@ void sift_lsh_state_free(sift_context* h) {
     if (!h || !h->lsh_state) return;
     auto* s = static_cast<LshScratch*>(h->lsh_state);
     for (DevBuf* b : {&s->q_u8, &s->q_nrm, &s->qcodes, &s->flag, &s->q_tiles.tiles, &s->q_tiles.norm,
-                      &s->q_tiles.norm_lane})
+                      &s->q_tiles.norm_lane, &s->fb_nc, &s->fb_list, &s->fb_count, &s->fb_map, &s->fb_out,
+                      &s->fb_q.tiles, &s->fb_q.norm, &s->fb_q.norm_lane})
         release(*b);
     delete s;
     h->lsh_state = nullptr;
@@ -347,41 +435,7 @@ int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int n
         if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = set_error(h, SIFT_ERR_HIP, "sync");
         release(hp_dev);
         if (rc != SIFT_OK) break;
-        // gather copy + exactness check
-        if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
-        const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-        if ((rc = pack_rows(h, r, type, r_n, rows_pad, KIND_BF16, PACK_PLAIN, h->b_set, flag)) != SIFT_OK) break;
-        if ((rc = check_flag(h, flag, "sift_lsh_build")) != SIFT_OK) break;
-        const size_t total = (size_t)r_n * L;
-        for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
-            if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
-        if (rc != SIFT_OK) break;
-        auto* keys = static_cast<uint32_t*>(ix->keys.p);
-        auto* vals = static_cast<uint32_t*>(ix->vals.p);
-        auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
-        auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
-        {
-            ProfScope ps(h, PROF_LSH_PROJ);
-            if ((rc = project(h, h->b_set, r_n, ix->hp, L, nbits, keys, r_n, true)) != SIFT_OK) break;
-        }
-        if (total) {
-            iota_mod<<<(unsigned)((total + 255) / 256), 256, 0, h->stream>>>(vals, (uint32_t)total, r_n);
-            if (hipGetLastError() != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "iota"); break; }
-        }
-        {
-            ProfScope ps(h, PROF_LSH_SORT);
-            if ((rc = radix_sort_pairs(h, &keys, &vals, &tk, &tv, (uint32_t)total, nbits + tbits, ix->hist,
-                                       ix->levels)) != SIFT_OK)
-                break;
-        }
-        // keep the sorted arrays in ix->keys / ix->vals
-        if (keys != ix->keys.p) {
-            std::swap(ix->keys, ix->scratch_k);
-            std::swap(ix->vals, ix->scratch_v);
-        }
-        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "sync"); break; }
-        release(ix->scratch_k);
-        release(ix->scratch_v);
+        rc = build_tables(h, ix, r, type, flag);
     } while (0);
     if (rc != SIFT_OK) {
         free_index(ix);
@@ -401,7 +455,7 @@ int sift_lsh_codes(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     int* flag = static_cast<int*>(sc->flag.p);
     const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
     SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, flag));
-    SIFT_TRY(project(h, sc->q_tiles, q_n, ix->hp, ix->L, ix->nbits, codes, q_n, false));
+    SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, codes, q_n, false));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return SIFT_OK;
 }
@@ -425,7 +479,7 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     auto* qcodes = static_cast<uint32_t*>(sc->qcodes.p);
     {
         ProfScope ps(h, PROF_LSH_PROJ);
-        SIFT_TRY(project(h, sc->q_tiles, q_n, ix->hp, ix->L, ix->nbits, qcodes, q_n, false));
+        SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, qcodes, q_n, false));
     }
     RerankArgs a;
     a.base_u8 = static_cast<const uint8_t*>(ix->base_u8.p);
@@ -449,6 +503,108 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     if (blocking) SIFT_TRY(check_flag(h, flag, "sift_lsh_query"));
+    return SIFT_OK;
+}
+
+int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_n, const sift_mlp_t* mlps,
+                           int L, sift_lsh_index_t* out) {
+    if (!h || !out || !mlps) return set_error(h, SIFT_ERR_INVALID, "null handle/index/networks");
+    *out = nullptr;
+    if (type < SIFT_FLOAT_HOST || type > SIFT_U8_DEVICE) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
+    if (L < 1 || L > 256) return set_error(h, SIFT_ERR_INVALID, "L must be in [1,256]");
+    for (int t = 0; t < L; t++)
+        if (!mlps[t]) return set_error(h, SIFT_ERR_INVALID, "null network");
+    const int nbits = mlp_nbits(mlps[0]);
+    for (int t = 1; t < L; t++)
+        if (mlp_nbits(mlps[t]) != nbits) return set_error(h, SIFT_ERR_INVALID, "networks differ in bits");
+    int tbits = 0;
+    while ((1 << tbits) < L) tbits++;
+    if (nbits + tbits > 32) return set_error(h, SIFT_ERR_UNSUPPORTED, "nbits + log2(L) must be <= 32");
+    if ((uint64_t)r_n * (uint64_t)L >= (1ull << 32)) return set_error(h, SIFT_ERR_UNSUPPORTED, "r_n * L too large");
+    if (r_n > 0 && !r) return set_error(h, SIFT_ERR_INVALID, "null base");
+    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+    auto* ix = new sift_lsh_index();
+    ix->h = h;
+    ix->n = r_n;
+    ix->nbits = nbits;
+    ix->L = L;
+    ix->hash_kind = SIFT_LSH_LEARNED;
+    ix->mlps.assign(mlps, mlps + L);
+    int rc = SIFT_OK;
+    do {
+        LshScratch* sc = scratch_of(h);
+        if ((rc = reserve(h, sc->flag, 16)) != SIFT_OK) break;
+        int* flag = static_cast<int*>(sc->flag.p);
+        if (hipMemsetAsync(flag, 0, 8, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
+        rc = build_tables(h, ix, r, type, flag);
+    } while (0);
+    if (rc != SIFT_OK) {
+        free_index(ix);
+        return rc;
+    }
+    *out = ix;
+    return SIFT_OK;
+}
+
+int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_t bf, const void* q, int type,
+                            uint32_t q_n, uint32_t min_cand, uint32_t* idx, float* dist, uint32_t* ncand,
+                            uint32_t* matches, float ratio, uint32_t* n_fallback) {
+    if (!h || !ix || !bf) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
+    if (bf->r_n != ix->n) return set_error(h, SIFT_ERR_INVALID, "the brute-force index holds another base");
+    if (n_fallback) *n_fallback = 0;
+    if (q_n == 0) return SIFT_OK;
+    if (!q) return set_error(h, SIFT_ERR_INVALID, "null queries");
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
+        type = SIFT_FLOAT_DEVICE;
+    }
+    LshScratch* sc = scratch_of(h);
+    SIFT_TRY(reserve(h, sc->fb_nc, (size_t)q_n * 4));
+    uint32_t* nc = ncand ? ncand : static_cast<uint32_t*>(sc->fb_nc.p);
+    SIFT_TRY(sift_lsh_query(h, ix, q, type, q_n, idx, dist, nc, matches, ratio, 0));
+    // queries with fewer than min_cand candidates, in query order
+    SIFT_TRY(reserve(h, sc->fb_list, (size_t)q_n * 4 + 4));
+    auto* list = static_cast<uint32_t*>(sc->fb_list.p);
+    SIFT_TRY(reserve(h, sc->fb_count, 4));
+    SIFT_HIP_TRY(h, hipMemsetAsync(sc->fb_count.p, 0, 4, h->stream));
+    select_low<<<(q_n + 255) / 256, 256, 0, h->stream>>>(nc, q_n, min_cand, static_cast<uint32_t*>(sc->fb_count.p),
+                                                         list);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    uint32_t cnt = 0;
+    SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (n_fallback) *n_fallback = cnt;
+    if (cnt == 0) return SIFT_OK;
+    // the list comes back in arbitrary order: sort it so the rerun is deterministic
+    std::vector<uint32_t> hl(cnt);
+    SIFT_HIP_TRY(h, hipMemcpy(hl.data(), list, (size_t)cnt * 4, hipMemcpyDeviceToHost));
+    std::sort(hl.begin(), hl.end());
+    const uint32_t rows_pad = query_rows_pad(bf->kind, cnt);
+    std::vector<int32_t> map(rows_pad, -1);
+    for (uint32_t i = 0; i < cnt; i++) map[i] = (int32_t)hl[i];
+    SIFT_TRY(reserve(h, sc->fb_map, (size_t)rows_pad * 4));
+    SIFT_HIP_TRY(h, hipMemcpy(sc->fb_map.p, map.data(), (size_t)rows_pad * 4, hipMemcpyHostToDevice));
+    // exact 2-NN of those queries against the packed base, then scatter back
+    PackedSet& qs = sc->fb_q;
+    SIFT_TRY(reserve(h, qs.tiles, (size_t)(rows_pad / 32) * tile_bytes(bf->kind)));
+    SIFT_TRY(reserve(h, qs.norm, (size_t)rows_pad * 4));
+    qs.rows = cnt;
+    qs.rows_pad = rows_pad;
+    qs.ntiles = rows_pad / 32;
+    qs.kind = bf->kind;
+    SIFT_TRY(reserve(h, sc->flag, 16));
+    SIFT_TRY(pack_launch(h, q, type, 0, static_cast<const int32_t*>(sc->fb_map.p), bf->kind, PACK_QUERY, qs,
+                         static_cast<int*>(sc->flag.p)));
+    SIFT_TRY(reserve(h, sc->fb_out, (size_t)cnt * 20));
+    auto* oi = static_cast<uint32_t*>(sc->fb_out.p);
+    auto* od = reinterpret_cast<float*>(oi + 2 * (size_t)cnt);
+    auto* om = oi + 4 * (size_t)cnt;
+    Knn2Out o{oi, od, nullptr, om, ratio};
+    SIFT_TRY(knn2_packed(h, qs, bf->base, cnt, bf->r_n, o));
+    scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(static_cast<const int32_t*>(sc->fb_map.p), cnt, oi, od,
+                                                            om, idx, dist, matches);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return SIFT_OK;
 }
 

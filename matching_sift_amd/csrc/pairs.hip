@@ -177,6 +177,7 @@ int build_tasks(sift_pairs_plan* p) {
     const long cap = (long)device_cu_count() * sh.pairs_blocks_per_cu();
     p->grid = (unsigned)std::max(1L, std::min<long>(cap, p->n_tasks));
     SIFT_TRY(reserve(h, p->tasks, std::max<size_t>(1, t.size()) * sizeof(PairTask)));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));  // no queued launch still reads the old list
     if (!t.empty())
         SIFT_HIP_TRY(h, hipMemcpy(p->tasks.p, t.data(), t.size() * sizeof(PairTask), hipMemcpyHostToDevice));
     return SIFT_OK;
@@ -530,6 +531,7 @@ int sift_pairs_plan_info(sift_pairs_plan_t p, int64_t* o) {
     o[7] = p->kind;
     // sticky: some packed value was not an integer in [0,255] (reads a device flag)
     int bad = 0;
+    if (p->h) (void)hipStreamSynchronize(p->h->stream);
     if (p->flag.p && hipMemcpy(&bad, p->flag.p, 4, hipMemcpyDeviceToHost) != hipSuccess) bad = -1;
     o[8] = bad;
     return SIFT_OK;

@@ -46,11 +46,17 @@ def _lib():
             "oracle_lsh_sort": [vp, i32, vp],
             "oracle_lsh_query": [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, i32],
             "oracle_mlp_forward": [vp, i32, vp, vp, i32, vp, vp, i32, vp, vp],
+            "oracle_hash_forward": [vp, i32, vp, i32, i32, vp, vp],
+            "oracle_hash_backward": [vp, i32, vp, i32, i32, vp, vp, vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = None
+        L.oracle_hash_loss.argtypes = [vp, vp, i32, i32, C.c_double, C.c_double, C.c_double]
+        L.oracle_hash_loss.restype = C.c_double
+        L.oracle_hash_loss_grad.argtypes = [vp, i32, i32, C.c_double, C.c_double, C.c_double, vp, vp]
+        L.oracle_hash_loss_grad.restype = C.c_double
         _L = L
     return _L
 
@@ -140,3 +146,44 @@ def mlp_forward(x, W1, b1, W2, b2):
     _lib().oracle_mlp_forward(x.ctypes.data, n, W1.ctypes.data, b1.ctypes.data, H, W2.ctypes.data,
                               b2.ctypes.data, B, hid.ctypes.data, out.ctypes.data)
     return hid, out
+
+
+HASH_NPARAM = 256 * 128 + 256 + 32 * 256 + 32
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def hash_forward(params, nbits, x, quant=True):
+    """Learned hash forward (library parameter layout): (hid[n,256], z[n,32])."""
+    params, x = _f32(params), _f32(x)
+    n = x.shape[0]
+    hid = np.empty((n, 256), np.float64)
+    z = np.empty((n, 32), np.float64)
+    _lib().oracle_hash_forward(params.ctypes.data, nbits, x.ctypes.data, n, 1 if quant else 0,
+                               hid.ctypes.data, z.ctypes.data)
+    return hid, z
+
+
+def hash_loss(s, z, B, nbits, alpha, beta, gamma):
+    s, z = _f64(s), _f64(z)
+    return _lib().oracle_hash_loss(s.ctypes.data, z.ctypes.data, B, nbits, alpha, beta, gamma)
+
+
+def hash_loss_grad(z, B, nbits, alpha, beta, gamma):
+    """(loss, dL/ds at s = sign z, dL/dz with the straight-through gradient)."""
+    z = _f64(z)
+    dls = np.empty_like(z)
+    dz = np.empty_like(z)
+    L = _lib().oracle_hash_loss_grad(z.ctypes.data, B, nbits, alpha, beta, gamma, dls.ctypes.data,
+                                     dz.ctypes.data)
+    return L, dls, dz
+
+
+def hash_backward(params, nbits, x, hid, dz, quant=True):
+    params, x = _f32(params), _f32(x)
+    g = np.empty(HASH_NPARAM, np.float64)
+    _lib().oracle_hash_backward(params.ctypes.data, nbits, x.ctypes.data, x.shape[0], 1 if quant else 0,
+                                _f64(hid).ctypes.data, _f64(dz).ctypes.data, g.ctypes.data)
+    return g

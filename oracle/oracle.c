@@ -253,3 +253,151 @@ void oracle_mlp_forward(const float* x, int n, const float* W1, const float* b1,
         }
     }
 }
+
+/* ------------------------------------------------------ learned hash -- */
+
+enum { HH = 256, HB = 32, P_W1 = 0, P_B1 = HH * 128, P_W2 = P_B1 + HH, P_B2 = P_W2 + HB * HH };
+
+/* bfloat16 round to nearest even (finite inputs), as the library's packers. */
+static float bf16_round(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+static double pw(const float* p, int i, int quant) { return quant ? (double)bf16_round(p[i]) : (double)p[i]; }
+/* layer 1 uses the stored weights times 1/64 (exact: a power of two) */
+static double pw1(const float* p, int i, int quant) {
+    const float w = p[i] * (1.f / 64.f);
+    return quant ? (double)bf16_round(w) : (double)w;
+}
+
+void oracle_hash_forward(const float* params, int nbits, const float* x, int n, int quant,
+                         double* hid, double* z) {
+    for (int i = 0; i < n; i++) {
+        for (int u = 0; u < HH; u++) {
+            double s = params[P_B1 + u];
+            for (int d = 0; d < 128; d++) {
+                const double xv = quant ? (double)bf16_round(x[(size_t)i * 128 + d]) : (double)x[(size_t)i * 128 + d];
+                s += pw1(params, P_W1 + u * 128 + d, quant) * xv;
+            }
+            double hv = s > 0.0 ? s : 0.0;
+            if (quant) hv = (double)bf16_round((float)hv);
+            hid[(size_t)i * HH + u] = hv;
+        }
+        for (int k = 0; k < HB; k++) {
+            double s = 0.0;
+            if (k < nbits) {
+                s = params[P_B2 + k];
+                for (int u = 0; u < HH; u++) s += pw(params, P_W2 + k * HH + u, quant) * hid[(size_t)i * HH + u];
+            }
+            z[(size_t)i * HB + k] = s;
+        }
+    }
+}
+
+double oracle_hash_loss(const double* s, const double* z, int B, int nbits, double alpha,
+                        double beta, double gamma) {
+    const int N = 2 * B;
+    const double b = nbits;
+    double sim = 0, bal = 0, dec = 0, q = 0;
+    for (int i = 0; i < B; i++)
+        for (int k = 0; k < nbits; k++) {
+            const double d = s[(size_t)i * HB + k] - s[(size_t)(i + B) * HB + k];
+            sim += d * d / 4.0;
+        }
+    for (int k = 0; k < nbits; k++) {
+        double m = 0;
+        for (int i = 0; i < N; i++) m += s[(size_t)i * HB + k];
+        m /= N;
+        bal += m * m;
+        for (int l = 0; l < nbits; l++) {
+            if (l == k) continue;
+            double c = 0;
+            for (int i = 0; i < N; i++) c += s[(size_t)i * HB + k] * s[(size_t)i * HB + l];
+            c /= N;
+            dec += c * c;
+        }
+    }
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < nbits; k++) {
+            const double a = fabs(z[(size_t)i * HB + k]) - 1.0;
+            q += a * a;
+        }
+    return sim / (B * b) + alpha * bal / b + beta * dec / (b * b) + gamma * q / (N * b);
+}
+
+double oracle_hash_loss_grad(const double* z, int B, int nbits, double alpha, double beta,
+                             double gamma, double* dls, double* dz) {
+    const int N = 2 * B;
+    const double b = nbits;
+    double* s = (double*)malloc(sizeof(double) * (size_t)N * HB);
+    double* mean = (double*)calloc(HB, sizeof(double));
+    double* cr = (double*)calloc(HB * HB, sizeof(double));
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < HB; k++) s[(size_t)i * HB + k] = z[(size_t)i * HB + k] >= 0.0 ? 1.0 : -1.0;
+    for (int k = 0; k < nbits; k++) {
+        for (int i = 0; i < N; i++) mean[k] += s[(size_t)i * HB + k];
+        mean[k] /= N;
+        for (int l = 0; l < nbits; l++) {
+            double c = 0;
+            for (int i = 0; i < N; i++) c += s[(size_t)i * HB + k] * s[(size_t)i * HB + l];
+            cr[k * HB + l] = c / N;
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        const int p = i < B ? i + B : i - B;
+        for (int k = 0; k < HB; k++) {
+            const size_t e = (size_t)i * HB + k;
+            if (k >= nbits) {
+                dls[e] = 0.0;
+                dz[e] = 0.0;
+                continue;
+            }
+            double g = (s[e] - s[(size_t)p * HB + k]) / (2.0 * B * b);
+            g += alpha * 2.0 * mean[k] / (b * N);
+            double dc = 0;
+            for (int l = 0; l < nbits; l++)
+                if (l != k) dc += cr[k * HB + l] * s[(size_t)i * HB + l];
+            g += beta * 4.0 * dc / (b * b * N);
+            dls[e] = g;
+            const double zv = z[e];
+            dz[e] = g + gamma * 2.0 * (fabs(zv) - 1.0) * s[e] / (N * b);
+        }
+    }
+    const double L = oracle_hash_loss(s, z, B, nbits, alpha, beta, gamma);
+    free(s);
+    free(mean);
+    free(cr);
+    return L;
+}
+
+void oracle_hash_backward(const float* params, int nbits, const float* x, int n, int quant,
+                          const double* hid, const double* dz, double* grads) {
+    memset(grads, 0, sizeof(double) * (size_t)(P_B2 + HB));
+    double* dh = (double*)malloc(sizeof(double) * HH);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < nbits; k++) {
+            const double g = dz[(size_t)i * HB + k];
+            grads[P_B2 + k] += g;
+            for (int u = 0; u < HH; u++) grads[P_W2 + k * HH + u] += g * hid[(size_t)i * HH + u];
+        }
+        for (int u = 0; u < HH; u++) {
+            double s = 0;
+            for (int k = 0; k < nbits; k++) s += dz[(size_t)i * HB + k] * pw(params, P_W2 + k * HH + u, quant);
+            dh[u] = hid[(size_t)i * HH + u] > 0.0 ? s : 0.0;
+        }
+        for (int u = 0; u < HH; u++) {
+            if (dh[u] == 0.0) continue;
+            grads[P_B1 + u] += dh[u];
+            for (int d = 0; d < 128; d++) {
+                const double xv = quant ? (double)bf16_round(x[(size_t)i * 128 + d]) : (double)x[(size_t)i * 128 + d];
+                grads[P_W1 + u * 128 + d] += dh[u] * xv / 64.0;
+            }
+        }
+    }
+    free(dh);
+}

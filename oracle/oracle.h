@@ -54,6 +54,32 @@ void oracle_lsh_query(const float* q, const float* r, int q_n, int r_n, int L,
 void oracle_mlp_forward(const float* x, int n, const float* W1, const float* b1, int H,
                         const float* W2, const float* b2, int B, double* hid, double* out);
 
+/* The library's learned hash (H = 256, outputs padded to 32), parameters in
+ * its flat layout: W1[256][128] (layer 1 multiplies by W1 / 64), b1[256],
+ * W2[32][256], b2[32].
+ * quant = 1 reproduces the GPU arithmetic's roundings: W1, W2 and the ReLU
+ * outputs rounded to bfloat16 (nearest even); accumulation in double.
+ * quant = 0: no rounding (a smooth function, for finite differences).
+ * hid: n x 256 (post-ReLU, as fed to layer 2), z: n x 32 (bits >= nbits = 0). */
+void oracle_hash_forward(const float* params, int nbits, const float* x, int n, int quant,
+                         double* hid, double* z);
+/* Loss of a batch of B pairs (rows i and B + i of s / z, N = 2B rows):
+ * sum (s_x - s_y)^2 / (4 B b) + alpha sum_k mean_k(s)^2 / b
+ * + beta sum_{k != l} (S^T S / N)_kl^2 / b^2 + gamma sum (|z| - 1)^2 / (N b),
+ * with s and z independent inputs (N x 32, bits >= nbits ignored). */
+double oracle_hash_loss(const double* s, const double* z, int B, int nbits, double alpha,
+                        double beta, double gamma);
+/* At s = sign(z) (sign(0) = +1): dls = dL/ds, and dz = dls (straight-through:
+ * the sign passes the gradient unchanged) plus the derivative of the gamma
+ * term; returns the loss. */
+double oracle_hash_loss_grad(const double* z, int B, int nbits, double alpha, double beta,
+                             double gamma, double* dls, double* dz);
+/* Gradient of sum_ik dz_ik * z_ik over the parameters (flat layout) given the
+ * forward's hid; quant as in oracle_hash_forward (rounded weights in the
+ * chain rule). */
+void oracle_hash_backward(const float* params, int nbits, const float* x, int n, int quant,
+                          const double* hid, const double* dz, double* grads);
+
 #ifdef __cplusplus
 }
 #endif

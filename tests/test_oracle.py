@@ -211,3 +211,102 @@ def test_mlp_forward_closed_form():
     h = np.maximum(x.astype(np.float64) @ W1.T.astype(np.float64) + b1, 0)
     np.testing.assert_allclose(hid, h, rtol=1e-12, atol=1e-12)
     np.testing.assert_allclose(out, h @ W2.T.astype(np.float64) + b2, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------ learned hash --
+
+def _hash_params(rng, nbits, scale=1.0):
+    p = np.zeros(oracle.HASH_NPARAM, np.float32)
+    p[:256 * 128] = rng.normal(0, 1.28 * scale, 256 * 128)
+    p[256 * 128:256 * 129] = rng.normal(0, 0.5, 256)
+    w2 = rng.normal(0, 0.06, (32, 256)).astype(np.float32)
+    w2[nbits:] = 0
+    p[256 * 129:256 * 129 + 32 * 256] = w2.ravel()
+    p[256 * 129 + 32 * 256:256 * 129 + 32 * 256 + nbits] = rng.normal(0, 0.1, nbits)
+    return p
+
+
+def test_hash_forward_matches_reference_forward():
+    """quant=0 equals the independent oracle_mlp_forward on the same weights;
+    quant=1 equals numpy with bfloat16 rounding of W1, W2, x and the hidden layer."""
+    rng = np.random.default_rng(11)
+    nbits = 20
+    p = _hash_params(rng, nbits)
+    x = rng.integers(0, 256, (7, 128)).astype(np.float32)
+    W1 = p[:256 * 128].reshape(256, 128) * np.float32(1 / 64)  # layer 1 uses the stored weights / 64
+    b1 = p[256 * 128:256 * 129]
+    W2 = p[256 * 129:256 * 129 + 32 * 256].reshape(32, 256)
+    b2 = p[256 * 129 + 32 * 256:]
+    hid, z = oracle.hash_forward(p, nbits, x, quant=False)
+    h2, o2 = oracle.mlp_forward(x, W1, b1, W2[:nbits], b2[:nbits])
+    np.testing.assert_allclose(hid, h2, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(z[:, :nbits], o2, rtol=1e-12, atol=1e-9)
+    assert (z[:, nbits:] == 0).all()
+
+    def bf(a):
+        u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+        u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+        return u.astype(np.uint32).view(np.float32).astype(np.float64)
+    hq = bf(np.maximum(x.astype(np.float64) @ bf(W1).T + b1, 0).astype(np.float32))
+    zq = hq @ bf(W2[:nbits]).T + b2[:nbits]
+    hid_q, z_q = oracle.hash_forward(p, nbits, x, quant=True)
+    np.testing.assert_allclose(hid_q, hq, rtol=0, atol=0)
+    np.testing.assert_allclose(z_q[:, :nbits], zq, rtol=1e-12, atol=1e-9)
+
+
+def test_hash_backward_finite_differences():
+    """The backward pass is the gradient of sum(G * z) (G fixed): central
+    differences on W1, b1, W2, b2 entries (no rounding: quant=0)."""
+    rng = np.random.default_rng(12)
+    nbits = 12
+    p = _hash_params(rng, nbits)
+    x = rng.integers(0, 256, (6, 128)).astype(np.float32)
+    G = rng.normal(size=(6, 32))
+    G[:, nbits:] = 0
+    hid, z = oracle.hash_forward(p, nbits, x, quant=False)
+    g = oracle.hash_backward(p, nbits, x, hid, G, quant=False)
+    f = lambda q: float((oracle.hash_forward(q, nbits, x, quant=False)[1] * G).sum())  # noqa: E731
+    picks = np.concatenate([rng.integers(0, 256 * 128, 12), rng.integers(256 * 128, 256 * 129, 6),
+                            256 * 129 + rng.integers(0, nbits, 6) * 256 + rng.integers(0, 256, 6),
+                            256 * 129 + 32 * 256 + rng.integers(0, nbits, 4)])
+    for i in picks:
+        eps = np.float32(2.0 ** -16)  # small enough that no ReLU input crosses zero
+        qp, qm = p.copy(), p.copy()
+        qp[i] += eps
+        qm[i] -= eps
+        fd = (f(qp) - f(qm)) / (float(qp[i]) - float(qm[i]))
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_hash_loss_closed_form_and_gradients():
+    """hash_loss equals its numpy statement; dL/ds and the |z| -> 1 term's
+    derivative match central differences; dL/dz is their straight-through
+    sum; a dropped term would fail the closed form."""
+    rng = np.random.default_rng(13)
+    B, nbits, al, be, ga = 8, 6, 0.7, 0.9, 0.3
+    z = rng.normal(0, 1.2, (2 * B, 32))
+    z[:, nbits:] = 0
+    s = np.where(z >= 0, 1.0, -1.0)
+    zs, ss = z[:, :nbits], s[:, :nbits]
+    N = 2 * B
+    C_ = ss.T @ ss / N
+    want = (((ss[:B] - ss[B:]) ** 2).sum() / 4 / (B * nbits) + al * (ss.mean(0) ** 2).sum() / nbits
+            + be * ((C_ ** 2).sum() - (np.diag(C_) ** 2).sum()) / nbits ** 2
+            + ga * ((np.abs(zs) - 1) ** 2).sum() / (N * nbits))
+    assert abs(oracle.hash_loss(s, z, B, nbits, al, be, ga) - want) < 1e-12
+    L, dls, dz = oracle.hash_loss_grad(z, B, nbits, al, be, ga)
+    assert abs(L - want) < 1e-12
+    eps = 1e-6
+    for _ in range(30):
+        i, k = rng.integers(0, N), rng.integers(0, nbits)
+        sp, sm = s.copy(), s.copy()
+        sp[i, k] += eps
+        sm[i, k] -= eps
+        fd = (oracle.hash_loss(sp, z, B, nbits, al, be, ga) - oracle.hash_loss(sm, z, B, nbits, al, be, ga)) / (2 * eps)
+        assert abs(fd - dls[i, k]) < 1e-7, (i, k, fd, dls[i, k])
+        zp, zm = z.copy(), z.copy()
+        zp[i, k] += eps
+        zm[i, k] -= eps
+        fdz = (oracle.hash_loss(s, zp, B, nbits, al, be, ga) - oracle.hash_loss(s, zm, B, nbits, al, be, ga)) / (2 * eps)
+        assert abs(dz[i, k] - dls[i, k] - fdz) < 1e-7, (i, k)  # straight-through + the gamma term
+    assert (dz[:, nbits:] == 0).all() and (dls[:, nbits:] == 0).all()
