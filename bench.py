@@ -199,6 +199,81 @@ def pairs_bench(args, ms, h, compute):
     print(json.dumps(out), flush=True)
 
 
+def learned_bench(args, ms, h):
+    """Learned hash against random-hyperplane LSH on the 1M x 10k shapes.
+    Each network (128 -> 256 -> b) is trained on 200,000 generated queries
+    labelled by the brute-force kernel; table t of a learned index uses
+    network t.  Both kinds of index answer with the exact brute-force fallback
+    for queries with fewer than 32 candidates.  One JSON line per (kind, b, L)
+    with recall of the exact 2-NN and queries/sec, then a summary line with
+    the best queries/sec of each kind at recall 0.8 and 0.95."""
+    from matching_sift_amd import datagen
+    M, N = args.m, args.n
+    log("generating base %d, queries %d, training queries %d" % (N, M, args.train_n))
+    b = datagen.base(N)
+    q = datagen.queries(b, M)
+    xtr = datagen.queries(b, args.train_n, seed=424242)
+    qd, bd, td = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b), ms.DeviceArray.from_numpy(xtr)
+    exact = ms.knn2(h, qd, bd, compute=ms.COMPUTE_BF16)[0]
+    bf = ms.BruteForceIndex(h, bd, compute=ms.COMPUTE_BF16, type_=ms.FLOAT_DEVICE)
+    outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
+            ms.DeviceArray.empty((M,), np.uint32), ms.DeviceArray.empty((M,), np.uint32))
+    bits = [int(x) for x in args.learned_bits.split(",")]
+    tables = [int(x) for x in args.learned_tables.split(",")]
+    rows = []
+
+    def measure(kind, nb, L, ix, build_s, train_s):
+        for _ in range(args.warmup):
+            ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            idx, _, nc, _, nf = ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE)
+        dt = (time.perf_counter() - t0) / args.steps
+        r1 = float((idx[:, 0] == exact[:, 0]).mean())
+        r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
+        row = {"workload": "learned", "kind": kind, "nbits": nb, "tables": L, "value": M / dt,
+               "unit": "queries/s", "recall_nn1": r1, "recall_2nn": r2, "mean_candidates": float(nc.mean()),
+               "fallback_queries": int(nf), "build_s": build_s, "train_s": train_s}
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+
+    for nb in bits:
+        nets, train_s = [], []
+        for t in range(max(L for L in tables if nb + max(0, (L - 1).bit_length()) <= 32)):
+            m = ms.HashMLP(h, nb, seed=1000 * nb + t)
+            t0 = time.perf_counter()
+            m.train(td, bd, epochs=args.train_epochs, batch=1024, lr=1e-3, seed=t, type_=ms.FLOAT_DEVICE)
+            train_s.append(time.perf_counter() - t0)
+            nets.append(m)
+            log("trained b=%d net %d in %.1f s" % (nb, t, train_s[-1]))
+        for L in tables:
+            if nb + max(0, (L - 1).bit_length()) > 32:  # table index and code share a 32-bit key
+                continue
+            h.sync()
+            t0 = time.perf_counter()
+            ix = ms.LSHIndex.learned(h, bd, nets[:L], type_=ms.FLOAT_DEVICE)
+            measure("learned", nb, L, ix, time.perf_counter() - t0, float(sum(train_s[:L])))
+            ix.close()
+            t0 = time.perf_counter()
+            ix = ms.LSHIndex(h, bd, nb, L, seed=7000 + nb, type_=ms.FLOAT_DEVICE)
+            measure("hyperplane", nb, L, ix, time.perf_counter() - t0, 0.0)
+            ix.close()
+        for m in nets:
+            m.close()
+    summary = {"workload": "learned-summary", "metric": "queries/s at equal recall of the exact 2-NN "
+               "(1M x 128 base, 10k queries, exact fallback below 32 candidates)"}
+    for kind in ("learned", "hyperplane"):
+        for rk in ("recall_2nn", "recall_nn1"):
+            for target in (0.8, 0.95):
+                ok = [r for r in rows if r["kind"] == kind and r[rk] >= target]
+                best = max(ok, key=lambda r: r["value"]) if ok else None
+                key = "%s_%s_%.2f" % (kind, rk, target)
+                summary[key] = ({"qps": best["value"], "nbits": best["nbits"], "tables": best["tables"],
+                                 rk: best[rk], "fallback_queries": best["fallback_queries"]} if best else None)
+    print(json.dumps(summary), flush=True)
+    bf.close()
+
+
 def copy_calibration(args):
     """A plain device copy of known size (torch's elementwise copy kernel):
     the reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
@@ -226,7 +301,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
-    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy"])
+    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy", "learned"])
+    ap.add_argument("--learned-bits", default="16,20,24")
+    ap.add_argument("--learned-tables", default="1,2,4,8,16")
+    ap.add_argument("--train-n", type=int, default=200_000)
+    ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--pairs", type=int, default=512)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--lsh-bits", default="8,12,16,20,24")
@@ -251,6 +330,9 @@ def main():
         return
     if args.workload == "pairs":
         pairs_bench(args, ms, ms.Handle(seed=1), compute)
+        return
+    if args.workload == "learned":
+        learned_bench(args, ms, ms.Handle(seed=1))
         return
     log("generating data N=%d M=%d" % (args.n, args.m))
     b = datagen.base(args.n)
