@@ -92,6 +92,27 @@ __device__ inline void barrier_only() {
     asm volatile("" ::: "memory");
 }
 
+// Stage hand-off counters in LDS (see fused_slab): wait until *c reaches
+// target, sleeping between polls.  The poll count is capped (about 2^16 x 64
+// cycles), so a wave can never wait forever.
+__device__ inline void wait_count(const uint32_t* c, uint32_t target) {
+    for (int it = 0; it < (1 << 16); it++) {
+        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    asm volatile("" ::: "memory");
+}
+// One increment per wave (lane 0), after everything before it in program order.
+__device__ inline void signal_count(uint32_t* c) {
+    asm volatile("" ::: "memory");
+    if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+// s_waitcnt lgkmcnt(0) alone (vmcnt and expcnt left at their maxima).
+__device__ inline void wait_lgkm0() {
+    __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14));
+    asm volatile("" ::: "memory");
+}
+
 // One (query group x base tile range) unit of work: the body of both the
 // grid-mapped kernel (one unit per block) and the persistent pairs kernel
 // (units from a task list).  Query index space: packed query rows; a query is
@@ -189,15 +210,33 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         return full;
     };
 
+    // Stage hand-off without block barriers.  Stage k lives in LDS slot
+    // k % 3.  s_ready[slot] counts the waves whose copies of that slot's
+    // current stage have landed; s_free[slot] the waves that have read it
+    // into registers.  A wave waits only for the stage it is about to read
+    // (every wave's copies landed) and for the slot it is about to refill
+    // (every wave has read the stage that used it), so a wave on the slow
+    // path holds the others back by at most one stage instead of stopping
+    // them at every stage.  Counters only grow within a unit of work.
+    __shared__ uint32_t s_ready[3], s_free[3];
+    if (threadIdx.x < 3) {
+        s_ready[threadIdx.x] = 0;
+        s_free[threadIdx.x] = 0;
+    }
+    barrier_only();
+
     const int nst = (t_end - t_begin + TPS - 1) / TPS;  // stages of this slab
     if (nst > 0) dma_stage(t_begin, 0);
     bool next_full = nst > 1 ? dma_stage(t_begin + TPS, 1) : false;
-    if (next_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
-    barrier_only();
-    int buf = 0;
+    if (nst > 0) {
+        if (next_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+        signal_count(&s_ready[0]);
+    }
     for (int k = 0; k < nst; k++) {
         const int ts = t_begin + k * TPS;
         const int ntile = min(TPS, t_end - ts);
+        const int buf = k % 3;
+        wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)));
         uint4 afr[TPS][G];
         int nlr[TPS][16];
         // (tiles past the slab end read stale LDS: in bounds, never flagged)
@@ -215,11 +254,17 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 }
             }
         }
-        // stage k+2 goes into the buffer consumed at stage k-1 (released by
-        // the barrier that closed stage k-1)
-        const int b2 = buf == 0 ? 2 : buf - 1;
+        wait_lgkm0();  // this stage's fragments are in registers
+        signal_count(&s_free[buf]);
+        // stage k+2 goes into the slot of stage k-1 once every wave has read it
+        const int b2 = (k + 2) % 3;
         const bool issued = k + 2 < nst;
+        if (issued && k >= 1) wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)));
         const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
+        if (k + 1 < nst) {  // this wave's copies of stage k+1 have landed
+            if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+            signal_count(&s_ready[(k + 1) % 3]);
+        }
 
         // Float kinds: the accumulator starts at the rows' |b|^2 (accumulator
         // lane order) and the queries are packed as -2q, so the MFMA chain
@@ -389,11 +434,6 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             }
         }
         }  // !COMP
-        if (k + 1 < nst) {  // stage k+1 complete for every wave; buffer buf free
-            if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
-            barrier_only();
-        }
-        buf = buf == 2 ? 0 : buf + 1;
     }
 
     if constexpr (COMP) {  // composites back to keys; pad rows (>= r_lim) are absent
@@ -557,14 +597,14 @@ int fused_occupancy() {
 
 const FusedShape kShapes[] = {
     // default per kind comes first
-    SHAPE(KIND_BF16, 2, 4, 2),
     SHAPE(KIND_BF16, 2, 8, 2),
+    SHAPE(KIND_BF16, 2, 4, 2),
     SHAPE(KIND_BF16, 2, 4, 4),
     SHAPE(KIND_BF16, 4, 4, 2),
     SHAPE(KIND_BF16, 1, 4, 2),
     SHAPE(KIND_BF16, 1, 8, 2),
-    SHAPE(KIND_F16, 2, 4, 2),
     SHAPE(KIND_F16, 2, 8, 2),
+    SHAPE(KIND_F16, 2, 4, 2),
     SHAPE(KIND_F32, 2, 4, 1),
     SHAPE(KIND_FP8, 2, 4, 2),
     SHAPE(KIND_FP8, 2, 8, 2),
