@@ -218,25 +218,30 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     // (every wave has read the stage that used it), so a wave on the slow
     // path holds the others back by at most one stage instead of stopping
     // them at every stage.  Counters only grow within a unit of work.
+    // Blocks of fewer than 8 waves keep one barrier per stage instead: their
+    // waves stay in step anyway, and the barrier is cheaper than the polls.
+    constexpr bool HANDOFF = WAVES >= 8;
     __shared__ uint32_t s_ready[3], s_free[3];
-    if (threadIdx.x < 3) {
-        s_ready[threadIdx.x] = 0;
-        s_free[threadIdx.x] = 0;
+    if constexpr (HANDOFF) {
+        if (threadIdx.x < 3) {
+            s_ready[threadIdx.x] = 0;
+            s_free[threadIdx.x] = 0;
+        }
+        barrier_only();
     }
-    barrier_only();
 
     const int nst = (t_end - t_begin + TPS - 1) / TPS;  // stages of this slab
     if (nst > 0) dma_stage(t_begin, 0);
     bool next_full = nst > 1 ? dma_stage(t_begin + TPS, 1) : false;
     if (nst > 0) {
         if (next_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
-        signal_count(&s_ready[0]);
+        if constexpr (HANDOFF) signal_count(&s_ready[0]); else barrier_only();
     }
     for (int k = 0; k < nst; k++) {
         const int ts = t_begin + k * TPS;
         const int ntile = min(TPS, t_end - ts);
         const int buf = k % 3;
-        wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)));
+        if constexpr (HANDOFF) wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)));
         uint4 afr[TPS][G];
         int nlr[TPS][16];
         // (tiles past the slab end read stale LDS: in bounds, never flagged)
@@ -254,16 +259,23 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 }
             }
         }
-        wait_lgkm0();  // this stage's fragments are in registers
-        signal_count(&s_free[buf]);
-        // stage k+2 goes into the slot of stage k-1 once every wave has read it
+        if constexpr (HANDOFF) {
+            wait_lgkm0();  // this stage's fragments are in registers
+            signal_count(&s_free[buf]);
+        }
+        // stage k+2 goes into the slot of stage k-1 once every wave has read
+        // it (hand-off: its counter; otherwise the barrier that closed stage k-1)
         const int b2 = (k + 2) % 3;
         const bool issued = k + 2 < nst;
-        if (issued && k >= 1) wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)));
+        if constexpr (HANDOFF) {
+            if (issued && k >= 1) wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)));
+        }
         const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
-        if (k + 1 < nst) {  // this wave's copies of stage k+1 have landed
-            if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
-            signal_count(&s_ready[(k + 1) % 3]);
+        if constexpr (HANDOFF) {
+            if (k + 1 < nst) {  // this wave's copies of stage k+1 have landed
+                if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+                signal_count(&s_ready[(k + 1) % 3]);
+            }
         }
 
         // Float kinds: the accumulator starts at the rows' |b|^2 (accumulator
@@ -434,6 +446,12 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             }
         }
         }  // !COMP
+        if constexpr (!HANDOFF) {
+            if (k + 1 < nst) {  // stage k+1 complete for every wave; this stage's slot free
+                if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
+                barrier_only();
+            }
+        }
     }
 
     if constexpr (COMP) {  // composites back to keys; pad rows (>= r_lim) are absent

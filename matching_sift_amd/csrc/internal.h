@@ -161,6 +161,10 @@ struct sift_context {
     Pending* pend = nullptr;
     int npend = 0, cap_pend = 0;
 
+    // launch shape picked by timing, per compute kind (index among the
+    // kind's listed shapes; -1 = not yet timed)
+    int tuned_shape[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+
     // LSH / learned-hash state lives in their own translation units
     void* lsh_state = nullptr;
 };

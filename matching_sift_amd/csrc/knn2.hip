@@ -200,13 +200,52 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
 
 }  // namespace
 
+// Bases of at least this many tiles pick their launch shape by timing: the
+// first call of a kind on a handle runs each of the kind's first two listed
+// shapes once (every shape gives identical results) and keeps the faster.
+// The best shape differs between boxes by up to 10% with no static rule.
+constexpr uint32_t TUNE_MIN_TILES = 8192;
+
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
                 uint32_t r_n, const Knn2Out& out) {
     if (qs.kind != bs.kind) return set_error(h, SIFT_ERR_INVALID, "query/base packed with different kinds");
     if (q_n == 0) return SIFT_OK;
     const FusedShape* sh = pick_shape(bs.kind);
     if (!sh) return set_error(h, SIFT_ERR_INVALID, "bad kind");
-    return launch_fused_shape(h, *sh, qs, bs, q_n, r_n, out);
+    const int kind = bs.kind;
+    if (std::getenv("SIFT_SHAPE") || bs.ntiles < TUNE_MIN_TILES || kind < 0 || kind >= 8)
+        return launch_fused_shape(h, *sh, qs, bs, q_n, r_n, out);
+    const FusedShape* cand[16];
+    const int nc = std::min(2, shapes_of_kind(kind, cand, 16));
+    if (h->tuned_shape[kind] >= 0 && h->tuned_shape[kind] < nc)
+        return launch_fused_shape(h, *cand[h->tuned_shape[kind]], qs, bs, q_n, r_n, out);
+    hipEvent_t e[2];
+    SIFT_HIP_TRY(h, hipEventCreate(&e[0]));
+    if (hipEventCreate(&e[1]) != hipSuccess) {
+        (void)hipEventDestroy(e[0]);
+        return set_error(h, SIFT_ERR_HIP, "hipEventCreate");
+    }
+    int best = 0, rc = SIFT_OK;
+    float best_ms = 3.4e38f;
+    for (int i = 0; i < nc && rc == SIFT_OK; i++) {
+        (void)hipEventRecord(e[0], h->stream);
+        rc = launch_fused_shape(h, *cand[i], qs, bs, q_n, r_n, out);
+        (void)hipEventRecord(e[1], h->stream);
+        if (rc != SIFT_OK) break;
+        float ms = 0.f;
+        if (hipEventSynchronize(e[1]) != hipSuccess || hipEventElapsedTime(&ms, e[0], e[1]) != hipSuccess) {
+            rc = set_error(h, SIFT_ERR_HIP, "shape timing");
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = i;
+        }
+    }
+    (void)hipEventDestroy(e[0]);
+    (void)hipEventDestroy(e[1]);
+    if (rc == SIFT_OK) h->tuned_shape[kind] = best;  // the last launch already wrote the outputs
+    return rc;
 }
 
 uint32_t query_rows_pad(int kind, uint32_t q_n) {

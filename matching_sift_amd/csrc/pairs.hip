@@ -144,15 +144,7 @@ int build_tasks(sift_pairs_plan* p) {
     sift_context* h = p->h;
     const FusedShape& sh = *p->shape;
     const uint32_t qpg = (uint32_t)(sh.qw * sh.waves * 32);
-    // Work items of one pair all go to one XCD, so its base image is fetched
-    // into that XCD's L2 once: block b runs on XCD b % 8 and takes items
-    // b, b + grid, ..., so item 8 i + x belongs to XCD x.  Pairs are dealt to
-    // the least loaded XCD, longest first; each XCD's list is sorted by
-    // decreasing length; lists are interleaved, short ones padded with empty
-    // items (no tiles, no queries).
-    constexpr int NX = 8;
-    std::vector<std::vector<PairTask>> per_pair(p->n_pairs);
-    std::vector<std::pair<double, uint32_t>> cost(p->n_pairs);
+    std::vector<PairTask> t;
     for (uint32_t i = 0; i < p->n_pairs; i++) {
         const uint32_t qi = p->pairs[2 * i], ri = p->pairs[2 * i + 1];
         const uint32_t nq = p->q_off[qi + 1] - p->q_off[qi];
@@ -161,7 +153,6 @@ int build_tasks(sift_pairs_plan* p) {
         // FP8: the base image in FP8_SLOTS quarters, each a candidate slot
         const int slots = p->kind == KIND_FP8 ? FP8_SLOTS : 1;
         const int32_t per = (nt + slots - 1) / slots;
-        double c = 0;
         for (uint32_t g = 0; g * qpg < nq; g++) {
             for (int sl = 0; sl < slots; sl++) {
                 PairTask w;
@@ -174,34 +165,16 @@ int build_tasks(sift_pairs_plan* p) {
                 w.q_count = nq;
                 w.out_off = p->out_off[i];
                 w.slot = sl;
-                per_pair[i].push_back(w);
-                c += 2.0 + (w.t_end - w.t_begin);  // a fixed per-item cost plus its tiles
+                t.push_back(w);
             }
         }
-        cost[i] = {c, i};
     }
-    std::stable_sort(cost.begin(), cost.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-    std::vector<std::vector<PairTask>> xl(NX);
-    std::vector<double> load(NX, 0.0);
-    for (const auto& ci : cost) {
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[x] += ci.first;
-        xl[x].insert(xl[x].end(), per_pair[ci.second].begin(), per_pair[ci.second].end());
-    }
-    size_t longest = 0;
-    for (auto& l : xl) {
-        std::stable_sort(l.begin(), l.end(), [](const PairTask& a, const PairTask& b) {
-            return a.t_end - a.t_begin > b.t_end - b.t_begin;
-        });
-        longest = std::max(longest, l.size());
-    }
-    PairTask empty{};
-    empty.q_count = 0;
-    std::vector<PairTask> t;
-    t.reserve(longest * NX);
-    for (size_t i = 0; i < longest; i++)
-        for (int x = 0; x < NX; x++) t.push_back(i < xl[x].size() ? xl[x][i] : empty);
-    while (!t.empty() && t.back().q_count == 0) t.pop_back();
+    // longest first, so the round-robin over a persistent grid ends evenly
+    // (grouping a pair's items on one XCD for L2 reuse measured slower: the
+    // kernel is bound by its top-2 epilogue, and the grouping unbalances it)
+    std::stable_sort(t.begin(), t.end(), [](const PairTask& a, const PairTask& b) {
+        return a.t_end - a.t_begin > b.t_end - b.t_begin;
+    });
     p->n_tasks = (int)t.size();
     const long cap = (long)device_cu_count() * sh.pairs_blocks_per_cu();
     p->grid = (unsigned)std::max(1L, std::min<long>(cap, p->n_tasks));
