@@ -164,7 +164,9 @@ struct FusedArgs {
                             // (Key::ord form), shared by all blocks; or null
     uint32_t q_n, q_stride;
     uint32_t r_n;           // real base rows; rows >= r_n in the last tile are masked
-    int n_tiles, tiles_per_slab, n_slabs, n_groups;
+    // brute-force kernel: slab s covers tiles [tile0 + s*tiles_per_slab, ...)
+    // clipped to end_tile (absolute tile numbers, so row indices are global)
+    int tile0, end_tile, tiles_per_slab, n_slabs, n_groups;
     // pairs kernel only
     const PairTask* tasks;
     int n_tasks;

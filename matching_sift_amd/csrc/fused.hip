@@ -539,8 +539,8 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     const int group = r % a.n_groups;
     if (slab >= a.n_slabs) return;  // uniform per block
     SlabTask t;
-    t.t_begin = slab * a.tiles_per_slab;
-    t.t_end = min(t.t_begin + a.tiles_per_slab, a.n_tiles);
+    t.t_begin = a.tile0 + slab * a.tiles_per_slab;
+    t.t_end = min(t.t_begin + a.tiles_per_slab, a.end_tile);
     t.qblk0 = group * WAVES * QW;
     t.r_lim = a.r_n;
     t.row_off = 0;

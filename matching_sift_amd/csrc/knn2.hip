@@ -30,22 +30,38 @@ struct MergeArgs {
     int* bound_out;       // seeding mode: write the shared 2nd-best bound instead
 };
 
-// One thread per query: lexicographic merge of the slab partials, then
-// d = key + ||q||^2 and the output formats.  In seeding mode the result is
-// the query's shared bound: its 2nd-best key over the sample rows.
+// Lexicographic merge of the slab partials, then d = key + ||q||^2 and the
+// output formats.  A block takes 64 queries; wave w merges slabs w, w + 8, ...
+// for them (each load is 1 KB contiguous), and wave 0 merges the 8 results.
+// In seeding mode the result is the query's shared bound: its 2nd-best key
+// over the rows merged.
+constexpr int MERGE_WAVES = 8;
 template <int KIND>
-__global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
+__global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.q_n) return;
+    __shared__ uint4 red[MERGE_WAVES - 1][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t q = blockIdx.x * 64u + lane;
     key_t m1 = Key<KIND>::none(), m2 = Key<KIND>::none();
     uint32_t i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
-    for (int s = 0; s < a.n_slabs; s++) {
-        const uint4 p = a.part[(size_t)s * a.q_stride + q];
+    if (q < a.q_n) {
+        for (int s = w; s < a.n_slabs; s += MERGE_WAVES) {
+            const uint4 p = a.part[(size_t)s * a.q_stride + q];
+            merge2(m1, i1, m2, i2, __builtin_bit_cast(key_t, p.x), p.y, __builtin_bit_cast(key_t, p.z), p.w);
+        }
+    }
+    if (w > 0) {
+        red[w - 1][lane] = make_uint4(__builtin_bit_cast(uint32_t, m1), i1, __builtin_bit_cast(uint32_t, m2), i2);
+    }
+    __syncthreads();
+    if (w > 0 || q >= a.q_n) return;
+#pragma unroll
+    for (int v = 0; v < MERGE_WAVES - 1; v++) {
+        const uint4 p = red[v][lane];
         merge2(m1, i1, m2, i2, __builtin_bit_cast(key_t, p.x), p.y, __builtin_bit_cast(key_t, p.z), p.w);
     }
     const bool has1 = i1 < a.r_n, has2 = i2 < a.r_n;
-    if (a.bound_out) {  // inclusive bound: the 2nd-best key over the sample
+    if (a.bound_out) {  // inclusive bound: the 2nd-best key over the rows merged
         a.bound_out[q] = Key<KIND>::ord(has2 ? m2 : Key<KIND>::none());
         return;
     }
@@ -84,9 +100,60 @@ __global__ __launch_bounds__(256) void merge_top2(MergeArgs a) {
     }
 }
 
-// Base rows scanned by the seeding pass (a bound per query from the first
-// SEED_TILES tiles); used when the base has many more tiles than that.
-constexpr int SEED_TILES = 128;
+// Seeding.  The base is processed as a chain of launches over growing tile
+// ranges [0, e0), [e0, e1), ..., [e_last, n): the first runs unbounded, and
+// each later one starts every lane from the bound B = the query's 2nd-best key
+// over all rows before its range (a merge of the earlier launches' partials),
+// so fewer rows pass the admission test and the slow path runs less.  Every
+// launch's partials stay in the final merge: no row is scanned twice.
+// Default ends: 128 tiles (4,096 rows), then x8 while the range is at most a
+// quarter of the base.  SIFT_SEED="e0,e1,..." (tile counts) overrides, "0"
+// disables seeding.
+int seed_ends(int n_tiles, int* ends, int max_ends) {
+    int n = 0;
+    if (const char* env = std::getenv("SIFT_SEED")) {
+        const char* c = env;
+        while (*c && n < max_ends) {
+            char* e = nullptr;
+            const long v = std::strtol(c, &e, 10);
+            if (e == c) break;
+            if (v <= 0) return 0;
+            if (v < n_tiles && (n == 0 || v > ends[n - 1])) ends[n++] = (int)v;
+            c = (*e == ',') ? e + 1 : e;
+            if (*e != ',') break;
+        }
+        return n;
+    }
+    for (long e = 128; 4 * e <= n_tiles && n < max_ends; e *= 8) ends[n++] = (int)e;
+    return n;
+}
+
+// Slab count for one launch over `n_tiles` tiles: blocks run in rounds of
+// `cap` resident blocks; pick the count whose block total fills its last
+// round best, preferring 2-8 rounds, counting a block's start-up (query
+// fragments, pipeline fill) as a few tiles.  Slabs are a multiple of 8 (one
+// XCD each).
+void choose_slabs(int n_tiles, int n_groups, int cap, int tps, int* n_slabs, int* tps_slab) {
+    constexpr double START_TILES = 6;
+    const int max_slabs = std::max(8, std::min(1024, ((n_tiles / tps + 7) / 8) * 8));
+    int best_ns = 8;
+    double best = 1e30;
+    for (int ns = 8; ns <= max_slabs; ns += 8) {
+        const int tps_s = ((n_tiles + ns - 1) / ns + tps - 1) / tps * tps;
+        const int used = (n_tiles + tps_s - 1) / tps_s;        // non-empty slabs
+        const long items = (long)n_groups * used;
+        const long rounds = (items + cap - 1) / cap;
+        double cost = (double)rounds * (tps_s + START_TILES);  // wall time ~ rounds x slab length
+        if (rounds < 2) cost *= 1.02;                          // keep some dynamic balance
+        cost *= 1.0 + 0.0005 * ns / 8;                         // the merge grows with slabs
+        if (cost < best) {
+            best = cost;
+            best_ns = ns;
+        }
+    }
+    *n_slabs = best_ns;
+    *tps_slab = std::max(tps, ((n_tiles + best_ns - 1) / best_ns + tps - 1) / tps * tps);
+}
 
 int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& qs,
                        const PackedSet& bs, uint32_t q_n, uint32_t r_n, const Knn2Out& out) {
@@ -96,62 +163,49 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
     const int n_tiles = (int)bs.ntiles;
     const uint32_t q_stride = (uint32_t)n_groups * qpg;
-    // Grid = query groups x base slabs (slabs a multiple of 8: one XCD each).
-    // Blocks run in rounds of `cap` resident blocks; pick the slab count whose
-    // block total fills its last round best, preferring 2-8 rounds.
     const int cap = device_cu_count() * sh.blocks_per_cu();
-    const int max_slabs = std::max(8, std::min(1024, ((n_tiles / sh.tps + 7) / 8) * 8));
-    int n_slabs = 8;
-    double best = 1e30;
-    for (int ns = 8; ns <= max_slabs; ns += 8) {
-        const int tps_s = ((n_tiles + ns - 1) / ns + sh.tps - 1) / sh.tps * sh.tps;
-        const int used = (n_tiles + tps_s - 1) / tps_s;        // non-empty slabs
-        const long items = (long)n_groups * used;
-        const long rounds = (items + cap - 1) / cap;
-        double cost = (double)rounds * tps_s;                  // wall time ~ rounds x slab length
-        if (rounds < 2) cost *= 1.02;                          // keep some dynamic balance
-        cost *= 1.0 + 0.0005 * ns / 8;                         // merge and seeding grow with slabs
-        if (cost < best) {
-            best = cost;
-            n_slabs = ns;
-        }
-    }
-    int tps_slab = ((n_tiles + n_slabs - 1) / n_slabs + sh.tps - 1) / sh.tps * sh.tps;
-    tps_slab = std::max(tps_slab, sh.tps);
 
-    const bool seed = r_n >= 2 && n_tiles >= 4 * SEED_TILES;
-    const int seed_tiles = SEED_TILES;
-    int seed_slab_tiles = ((seed_tiles / 8 + sh.tps - 1) / sh.tps) * sh.tps;
-    const int seed_slabs = seed ? (((seed_tiles + seed_slab_tiles - 1) / seed_slab_tiles + 7) / 8) * 8 : 0;
-    SIFT_TRY(reserve(h, h->part, (size_t)(n_slabs + seed_slabs) * q_stride * sizeof(uint4)));
+    // launch l covers tiles [t0[l], t0[l + 1]); its partials start at slab p0[l]
+    constexpr int MAX_LAUNCH = 8;
+    int t0[MAX_LAUNCH + 1], slabs[MAX_LAUNCH], tps_slab[MAX_LAUNCH], p0[MAX_LAUNCH + 1];
+    int n_launch = 1 + (r_n >= 2 ? seed_ends(n_tiles, t0 + 1, MAX_LAUNCH - 1) : 0);
+    t0[0] = 0;
+    t0[n_launch] = n_tiles;
+    p0[0] = 0;
+    for (int l = 0; l < n_launch; l++) {
+        choose_slabs(t0[l + 1] - t0[l], n_groups, cap, sh.tps, &slabs[l], &tps_slab[l]);
+        p0[l + 1] = p0[l] + slabs[l];
+    }
+    SIFT_TRY(reserve(h, h->part, (size_t)p0[n_launch] * q_stride * sizeof(uint4)));
     SIFT_TRY(reserve(h, h->thresh, (size_t)q_stride * sizeof(int)));
     uint4* part = static_cast<uint4*>(h->part.p);
     int* bound = static_cast<int*>(h->thresh.p);
 
-    auto mk_fused = [&](int nt, int per_slab, int slabs, uint4* dst, int* bnd) {
-        FusedArgs fa;
+    auto mk_fused = [&](int l) {
+        FusedArgs fa{};
         fa.q_tiles = static_cast<const uint4*>(qs.tiles.p);
         fa.b_tiles = static_cast<const uint4*>(bs.tiles.p);
         fa.b_nl = static_cast<const float*>(bs.norm_lane.p);
-        fa.part = dst;
-        fa.bound = bnd;
+        fa.part = part + (size_t)p0[l] * q_stride;
+        fa.bound = l > 0 ? bound : nullptr;
         fa.q_n = q_n;
         fa.r_n = r_n;
         fa.q_stride = q_stride;
-        fa.n_tiles = nt;
-        fa.tiles_per_slab = per_slab;
-        fa.n_slabs = slabs;
+        fa.tile0 = t0[l];
+        fa.end_tile = t0[l + 1];
+        fa.tiles_per_slab = tps_slab[l];
+        fa.n_slabs = slabs[l];
         fa.n_groups = n_groups;
         return fa;
     };
-    auto mk_merge = [&](const uint4* src, int slabs) {
+    auto mk_merge = [&](int n_part) {
         MergeArgs ma;
-        ma.part = src;
+        ma.part = part;
         ma.q_norm = static_cast<const float*>(qs.norm.p);
         ma.q_n = q_n;
         ma.r_n = r_n;
         ma.q_stride = q_stride;
-        ma.n_slabs = slabs;
+        ma.n_slabs = n_part;
         ma.idx = out.idx;
         ma.dist = out.dist;
         ma.sorted = out.sorted;
@@ -161,38 +215,30 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         return ma;
     };
     auto launch_merge = [&](const MergeArgs& ma) {
-        const unsigned g = (q_n + 255) / 256;
+        const unsigned g = (q_n + 63) / 64, b = 64 * MERGE_WAVES;
         switch (bs.kind) {
-            case KIND_BF16: merge_top2<KIND_BF16><<<g, 256, 0, h->stream>>>(ma); break;
-            case KIND_F16: merge_top2<KIND_F16><<<g, 256, 0, h->stream>>>(ma); break;
-            case KIND_F32: merge_top2<KIND_F32><<<g, 256, 0, h->stream>>>(ma); break;
-            default: merge_top2<KIND_I8><<<g, 256, 0, h->stream>>>(ma); break;
+            case KIND_BF16: merge_top2<KIND_BF16><<<g, b, 0, h->stream>>>(ma); break;
+            case KIND_F16: merge_top2<KIND_F16><<<g, b, 0, h->stream>>>(ma); break;
+            case KIND_F32: merge_top2<KIND_F32><<<g, b, 0, h->stream>>>(ma); break;
+            default: merge_top2<KIND_I8><<<g, b, 0, h->stream>>>(ma); break;
         }
     };
 
-    int* bnd = nullptr;
-    if (seed) {
-        uint4* sp = part + (size_t)n_slabs * q_stride;
-        FusedArgs fa = mk_fused(seed_tiles, seed_slab_tiles, seed_slabs, sp, nullptr);
-        {
+    for (int l = 0; l < n_launch; l++) {
+        if (l > 0) {  // bound from everything before this range
             ProfScope ps(h, PROF_SEED);
-            sh.fn(fa, (unsigned)(seed_slabs * n_groups), h->stream);
-            SIFT_HIP_TRY(h, hipGetLastError());
-            MergeArgs ma = mk_merge(sp, seed_slabs);
+            MergeArgs ma = mk_merge(p0[l]);
             ma.bound_out = bound;
             launch_merge(ma);
             SIFT_HIP_TRY(h, hipGetLastError());
         }
-        bnd = bound;
-    }
-    {
         ProfScope ps(h, PROF_FUSED);
-        sh.fn(mk_fused(n_tiles, tps_slab, n_slabs, part, bnd), (unsigned)(n_slabs * n_groups), h->stream);
+        sh.fn(mk_fused(l), (unsigned)(slabs[l] * n_groups), h->stream);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     {
         ProfScope ps(h, PROF_MERGE);
-        launch_merge(mk_merge(part, n_slabs));
+        launch_merge(mk_merge(p0[n_launch]));
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     return SIFT_OK;

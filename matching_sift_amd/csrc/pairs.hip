@@ -195,7 +195,8 @@ FusedArgs make_args(sift_pairs_plan* p, uint32_t* idx, float* dist, uint32_t* ma
     a.q_n = 0;
     a.q_stride = 0;
     a.r_n = 0;
-    a.n_tiles = (int)p->bs.ntiles;
+    a.tile0 = 0;
+    a.end_tile = (int)p->bs.ntiles;
     a.tiles_per_slab = 0;
     a.n_slabs = 0;
     a.n_groups = 0;
