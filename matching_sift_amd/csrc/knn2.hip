@@ -106,8 +106,8 @@ __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
 // over all rows before its range (a merge of the earlier launches' partials),
 // so fewer rows pass the admission test and the slow path runs less.  Every
 // launch's partials stay in the final merge: no row is scanned twice.
-// Default ends: 128 tiles (4,096 rows), then x8 while the range is at most a
-// quarter of the base.  SIFT_SEED="e0,e1,..." (tile counts) overrides, "0"
+// Default ends: 128 tiles (4,096 rows), then x8 while the end is at most a
+// third of the base.  SIFT_SEED="e0,e1,..." (tile counts) overrides, "0"
 // disables seeding.
 int seed_ends(int n_tiles, int* ends, int max_ends) {
     int n = 0;
@@ -124,7 +124,7 @@ int seed_ends(int n_tiles, int* ends, int max_ends) {
         }
         return n;
     }
-    for (long e = 128; 4 * e <= n_tiles && n < max_ends; e *= 8) ends[n++] = (int)e;
+    for (long e = 128; 3 * e <= n_tiles && n < max_ends; e *= 8) ends[n++] = (int)e;
     return n;
 }
 

@@ -178,9 +178,9 @@ SHAPES = {COMPUTE_BF16: ["2,4,2", "2,4,4", "2,8,2", "4,4,2", "1,4,2", "1,8,2"],
 
 @pytest.mark.parametrize("compute,shape", [(c, s) for c in SHAPES for s in SHAPES[c]])
 def test_seeded_bound_all_shapes(handle, compute, shape, monkeypatch):
-    """Bases above 16,384 rows take the seeded-bound path (a per-query bound
-    from the first 4,096 rows); exercise it with realistic data and with a
-    3-letter alphabet where many rows tie exactly with the bound."""
+    """Bases of at least 12,288 rows take the seeded chain (the first launch
+    covers 4,096 rows and bounds the rest); exercise it with realistic data and
+    with a 3-letter alphabet where many rows tie exactly with the bound."""
     monkeypatch.setenv("SIFT_SHAPE", shape)
     say("seeded", compute, shape)
     b = datagen.base(40_000, seed=71)
@@ -197,3 +197,48 @@ def test_seeded_bound_all_shapes(handle, compute, shape, monkeypatch):
     s, mt = expected(qq, r)
     idx, dist, srt, mat = knn2(handle, qq, r, compute=compute, ratio=0.8)
     np.testing.assert_array_equal(srt, s)
+
+
+@pytest.mark.parametrize("compute,shape", [(COMPUTE_BF16, "2,8,2"), (COMPUTE_BF16, "2,4,2"),
+                                           (COMPUTE_F16, "2,4,2"), (COMPUTE_I8, "2,4,2"),
+                                           (COMPUTE_F32, "2,4,1")])
+@pytest.mark.parametrize("levels", ["0", "3,17,200", "16,64,256,1000", "128,1024"])
+def test_seed_chain_levels(handle, compute, shape, levels, monkeypatch):
+    """The base is scanned as a chain of launches over consecutive tile
+    ranges, each starting from the bound of the ranges before it, and every
+    launch's partials join the final merge.  Any chain (none, unaligned range
+    ends, many short ranges) gives the oracle's result exactly, also when
+    duplicates of early rows sit in later ranges and tie with the bound."""
+    monkeypatch.setenv("SIFT_SHAPE", shape)
+    monkeypatch.setenv("SIFT_SEED", levels)
+    say("seed chain", compute, shape, levels)
+    b = datagen.base(40_000, seed=91)
+    q = datagen.queries(b, 260, seed=92)
+    s, mt = expected(q, b)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+    rng = np.random.default_rng(9)
+    r = rng.integers(0, 3, size=(36_000, 128)).astype(np.float32)
+    for lo in (90, 600, 6_000, 31_000, 33_000):  # copies of rows of the first range in every later one
+        r[lo:lo + 40] = r[10:50]
+    qq = rng.integers(0, 3, size=(130, 128)).astype(np.float32)
+    qq[:40] = r[10:50]
+    s, mt = expected(qq, r)
+    idx, dist, srt, mat = knn2(handle, qq, r, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+
+
+def test_full_size_default_chain(handle):
+    """The benchmark's base (1,000,000 rows: a chain of four launches over
+    128, 896, 7,168 and 23,058 tiles) against the oracle on a query sample
+    that includes exact copies of base rows in every range."""
+    say("full size 1M")
+    b = datagen.base(1_000_000)
+    q = datagen.queries(b, 96, seed=93)
+    q[:4] = b[[5, 9_000, 100_000, 999_999]]
+    s, mt = expected(q, b)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=COMPUTE_BF16, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
