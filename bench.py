@@ -399,7 +399,7 @@ def main():
         try:
             tj = json.load(open(args.traffic_json))
             if tj.get("workload") == "%dx%d-%s" % (N, M, args.compute):
-                traffic = tj.get("bytes_per_launch")
+                traffic = tj.get("bytes_per_step")  # summed over the step's fused_top2 launches
         except Exception:
             traffic = None
 
