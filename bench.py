@@ -206,9 +206,13 @@ def learned_bench(args, ms, h):
     Each network (128 -> 256 -> b) is trained on 200,000 generated queries
     labelled by the brute-force kernel; table t of a learned index uses
     network t.  Both kinds of index answer with the exact brute-force fallback
-    for queries with fewer than 32 candidates.  One JSON line per (kind, b, L)
-    with recall of the exact 2-NN and queries/sec, then a summary line with
-    the best queries/sec of each kind at recall 0.8 and 0.95."""
+    for queries with fewer than 32 candidates ("count" rule), and in a second
+    measurement also for queries whose best candidate fails the ratio test
+    0.8 against the next one ("ratio" rule: the hash answers the confident
+    matches, the exact path the ambiguous ones).  One JSON line per (rule,
+    kind, b, L) with recall of the exact NN1, 2-NN and match output, and
+    queries/sec; then a summary line with the best queries/sec of each kind
+    at recall 0.8 and 0.95, beside the exact brute-force index alone."""
     from matching_sift_amd import datagen
     M, N = args.m, args.n
     log("generating base %d, queries %d, training queries %d" % (N, M, args.train_n))
@@ -216,8 +220,18 @@ def learned_bench(args, ms, h):
     q = datagen.queries(b, M)
     xtr = datagen.queries(b, args.train_n, seed=424242)
     qd, bd, td = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b), ms.DeviceArray.from_numpy(xtr)
-    exact = ms.knn2(h, qd, bd, compute=ms.COMPUTE_BF16)[0]
+    ex = ms.knn2(h, qd, bd, compute=ms.COMPUTE_BF16, ratio=0.8)
+    exact, exact_m = ex[0], ex[3]
     bf = ms.BruteForceIndex(h, bd, compute=ms.COMPUTE_BF16, type_=ms.FLOAT_DEVICE)
+    bo = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32), None, None)
+    for _ in range(args.warmup):
+        bf.query(qd, outputs=bo, type_=ms.FLOAT_DEVICE, blocking=False)
+    h.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bf.query(qd, outputs=bo, type_=ms.FLOAT_DEVICE, blocking=False)
+    h.sync()
+    brute_qps = M * args.steps / (time.perf_counter() - t0)
     outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
             ms.DeviceArray.empty((M,), np.uint32), ms.DeviceArray.empty((M,), np.uint32))
     bits = [int(x) for x in args.learned_bits.split(",")]
@@ -225,19 +239,30 @@ def learned_bench(args, ms, h):
     rows = []
 
     def measure(kind, nb, L, ix, build_s, train_s):
-        for _ in range(args.warmup):
-            ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE)
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            idx, _, nc, _, nf = ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE)
-        dt = (time.perf_counter() - t0) / args.steps
-        r1 = float((idx[:, 0] == exact[:, 0]).mean())
-        r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
-        row = {"workload": "learned", "kind": kind, "nbits": nb, "tables": L, "value": M / dt,
-               "unit": "queries/s", "recall_nn1": r1, "recall_2nn": r2, "mean_candidates": float(nc.mean()),
-               "fallback_queries": int(nf), "build_s": build_s, "train_s": train_s}
-        rows.append(row)
-        print(json.dumps(row), flush=True)
+        for rule, acc in (("count", 0.0), ("ratio", 0.8)):
+            for _ in range(args.warmup):
+                ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE, accept_ratio=acc)
+            h.sync()
+            h.profile(True)
+            h.profile_reset()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                idx, _, nc, mat, nf = ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE,
+                                                        accept_ratio=acc)
+            dt = (time.perf_counter() - t0) / args.steps
+            # device time per call: hashing, bucket lookup + re-rank, exact fallback kernels
+            brk = {k: h.profile_read(p)[0] / args.steps for k, p in
+                   (("hash_ms", "lsh_proj"), ("rerank_ms", "lsh_query"), ("fallback_fused_ms", "fused"))}
+            h.profile(False)
+            r1 = float((idx[:, 0] == exact[:, 0]).mean())
+            r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
+            rm = float((mat == exact_m).mean())
+            row = {"workload": "learned", "rule": rule, "kind": kind, "nbits": nb, "tables": L, "value": M / dt,
+                   "unit": "queries/s", "recall_nn1": r1, "recall_2nn": r2, "recall_match": rm,
+                   "mean_candidates": float(nc.mean()), "fallback_queries": int(nf), "build_s": build_s,
+                   "train_s": train_s, "ms_per_call": dt * 1e3, "breakdown": brk}
+            rows.append(row)
+            print(json.dumps(row), flush=True)
 
     for nb in bits:
         nets, train_s = [], []
@@ -262,16 +287,18 @@ def learned_bench(args, ms, h):
             ix.close()
         for m in nets:
             m.close()
-    summary = {"workload": "learned-summary", "metric": "queries/s at equal recall of the exact 2-NN "
-               "(1M x 128 base, 10k queries, exact fallback below 32 candidates)"}
-    for kind in ("learned", "hyperplane"):
-        for rk in ("recall_2nn", "recall_nn1"):
-            for target in (0.8, 0.95):
-                ok = [r for r in rows if r["kind"] == kind and r[rk] >= target]
-                best = max(ok, key=lambda r: r["value"]) if ok else None
-                key = "%s_%s_%.2f" % (kind, rk, target)
-                summary[key] = ({"qps": best["value"], "nbits": best["nbits"], "tables": best["tables"],
-                                 rk: best[rk], "fallback_queries": best["fallback_queries"]} if best else None)
+    summary = {"workload": "learned-summary", "metric": "queries/s at equal recall of the exact result "
+               "(1M x 128 base, 10k queries, exact fallback below 32 candidates; ratio rule: also when the "
+               "best candidate fails the ratio test 0.8)", "brute_force_index_qps": brute_qps}
+    for rule in ("count", "ratio"):
+        for kind in ("learned", "hyperplane"):
+            for rk in ("recall_2nn", "recall_nn1", "recall_match"):
+                for target in (0.8, 0.95):
+                    ok = [r for r in rows if r["rule"] == rule and r["kind"] == kind and r[rk] >= target]
+                    best = max(ok, key=lambda r: r["value"]) if ok else None
+                    key = "%s_%s_%s_%.2f" % (rule, kind, rk, target)
+                    summary[key] = ({"qps": best["value"], "nbits": best["nbits"], "tables": best["tables"],
+                                     rk: best[rk], "fallback_queries": best["fallback_queries"]} if best else None)
     print(json.dumps(summary), flush=True)
     bf.close()
 

@@ -262,11 +262,14 @@ int sift_mlp_train(sift_handle_t h, sift_mlp_t mlp, const void* x, uint32_t n, c
 int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_n,
                            const sift_mlp_t* mlps, int L, sift_lsh_index_t* index);
 /* sift_lsh_query, then exact brute force (bf: the same base, packed) for the
- * queries with fewer than min_cand candidates; n_fallback (host) = how many. */
+ * queries the LSH answer does not settle: fewer than min_cand candidates, or,
+ * when accept_ratio > 0, a best candidate failing sqrt(d1)/sqrt(d2) <
+ * accept_ratio against the next candidate (0 = count rule only; at most 1).
+ * n_fallback (host) = how many.  No counterpart in the reference. */
 int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t index, sift_bf_index_t bf,
                             const void* q_points, int type, uint32_t q_n, uint32_t min_cand,
-                            uint32_t* idx, float* dist, uint32_t* ncand, uint32_t* matches,
-                            float ratio, uint32_t* n_fallback);
+                            float accept_ratio, uint32_t* idx, float* dist, uint32_t* ncand,
+                            uint32_t* matches, float ratio, uint32_t* n_fallback);
 
 typedef struct sift_pairs_plan* sift_pairs_plan_t;
 #define SIFT_PAIRS_GRAPH 1u    /* capture the execute sequence in a HIP graph (replayed while

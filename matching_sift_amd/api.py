@@ -199,9 +199,11 @@ class LSHIndex:
         self.ix = ix.value
         return self
 
-    def query_fallback(self, q, bf_index, min_cand=32, ratio=0.8, type_=None, outputs=None):
-        """Query, then exact brute force for queries with fewer than min_cand
-        candidates.  Returns (idx, dist, ncand, matches, n_fallback)."""
+    def query_fallback(self, q, bf_index, min_cand=32, ratio=0.8, type_=None, outputs=None, accept_ratio=0.0):
+        """Query, then exact brute force for the queries with fewer than
+        min_cand candidates or (accept_ratio > 0) whose best candidate fails
+        the ratio test sqrt(d1)/sqrt(d2) < accept_ratio against the next one.
+        Returns (idx, dist, ncand, matches, n_fallback)."""
         qd = to_device(q)
         M = qd.shape[0]
         if outputs is None:
@@ -210,7 +212,8 @@ class LSHIndex:
         idx, dist, nc, mat = outputs
         nf = C.c_uint32()
         check(lib().sift_lsh_query_fallback(self.handle.h, self.ix, bf_index.ix, qd.ptr, _input_type(qd, type_), M,
-                                            int(min_cand), idx.ptr, dist.ptr, nc.ptr, mat.ptr, float(ratio),
+                                            int(min_cand), float(accept_ratio), idx.ptr, dist.ptr, nc.ptr,
+                                            mat.ptr, float(ratio),
                                             C.byref(nf)), self.handle.h)
         return idx.numpy(), dist.numpy(), nc.numpy(), mat.numpy(), nf.value
 

@@ -46,6 +46,7 @@ struct sift_lsh_index {
     PackedSet hp;               // hyperplanes packed as B operands, 32 per table
     std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller)
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
+    DevBuf offs;                // b <= OFFS_MAX_BITS: bucket starts, [L][2^b + 1] absolute positions
     DevBuf base_u8, base_nrm;   // gather copy
     DevBuf scratch_k, scratch_v, hist, levels[4];
 };
@@ -57,7 +58,7 @@ struct LshScratch {
     DevBuf q_u8, q_nrm, qcodes, flag;
     PackedSet q_tiles;
     // exact fallback
-    DevBuf fb_nc, fb_list, fb_count, fb_map, fb_out;
+    DevBuf fb_nc, fb_dist, fb_list, fb_count, fb_map, fb_out;
     PackedSet fb_q;
 };
 
@@ -111,10 +112,18 @@ __global__ void iota_mod(uint32_t* __restrict__ v, uint32_t total, uint32_t n) {
 
 // Queries whose candidate count is below min_cand (order of the list is
 // arbitrary; the host sorts it).
-__global__ void select_low(const uint32_t* __restrict__ nc, uint32_t q_n, uint32_t min_cand,
-                           uint32_t* __restrict__ count, uint32_t* __restrict__ list) {
+// Queries the LSH answer does not settle: fewer than min_cand candidates, or
+// (accept_ratio > 0) a best candidate that does not pass the ratio test
+// sqrt(d1)/sqrt(d2) < accept_ratio against the next one (IEEE sqrt and
+// division, as the match output).
+__global__ void select_low(const uint32_t* __restrict__ nc, const float* __restrict__ d, uint32_t q_n,
+                           uint32_t min_cand, float accept_ratio, uint32_t* __restrict__ count,
+                           uint32_t* __restrict__ list) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < q_n && nc[q] < min_cand) list[atomicAdd(count, 1u)] = q;
+    if (q >= q_n) return;
+    bool low = nc[q] < min_cand;
+    if (accept_ratio > 0.f) low |= !(sqrtf(d[2 * (size_t)q]) / sqrtf(d[2 * (size_t)q + 1]) < accept_ratio);
+    if (low) list[atomicAdd(count, 1u)] = q;
 }
 
 // Exact results of the fallback queries into the caller's outputs.
@@ -143,6 +152,7 @@ struct RerankArgs {
     const uint32_t* keys;
     const uint32_t* vals;
     const uint32_t* qcodes;  // [L][q_n]
+    const uint32_t* offs;    // bucket starts [L][2^b + 1], or null (binary search)
     uint32_t n, q_n;
     int L, b;
     uint32_t* idx;
@@ -201,10 +211,18 @@ __global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
     uint32_t nc = 0;
     for (int t = 0; t < a.L; t++) {
         const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
-        const uint32_t key = tkey | a.qcodes[(size_t)t * a.q_n + q];
-        const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
-        const uint32_t lo = lower_bound_u32(a.keys, t0, t1, key);
-        const uint32_t hi = upper_bound_u32(a.keys, lo, t1, key);
+        const uint32_t code = a.qcodes[(size_t)t * a.q_n + q];
+        uint32_t lo, hi;
+        if (a.offs) {
+            const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
+            lo = o[0];
+            hi = o[1];
+        } else {
+            const uint32_t key = tkey | code;
+            const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
+            lo = lower_bound_u32(a.keys, t0, t1, key);
+            hi = upper_bound_u32(a.keys, lo, t1, key);
+        }
         nc += hi - lo;
         for (uint32_t p = lo + g; p < hi; p += 8) {
             const uint32_t j = a.vals[p];
@@ -310,7 +328,7 @@ int hash_codes(sift_context* h, const sift_lsh_index* ix, const PackedSet& xs, u
 }
 
 void free_index(sift_lsh_index* ix) {
-    for (DevBuf* b : {&ix->keys, &ix->vals, &ix->base_u8, &ix->base_nrm, &ix->scratch_k,
+    for (DevBuf* b : {&ix->keys, &ix->vals, &ix->offs, &ix->base_u8, &ix->base_nrm, &ix->scratch_k,
                       &ix->scratch_v, &ix->hist, &ix->hp.tiles, &ix->hp.norm, &ix->hp.norm_lane})
         release(*b);
     for (auto& l : ix->levels) release(l);
@@ -325,8 +343,32 @@ int check_flag(sift_context* h, int* flag, const char* what) {
     return SIFT_OK;
 }
 
+// Bucket starts for direct lookup: offs[t][c] = first position of table t's
+// sorted range [t*n, (t+1)*n) whose key is >= (t << b | c); offs[t][2^b] is
+// the table's end.  One thread per (table, code), each a binary search.
+constexpr int OFFS_MAX_BITS = 20;
+__global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, int b, uint32_t L,
+                               uint32_t* __restrict__ offs) {
+    const uint32_t per = (1u << b) + 1u;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)L * per) return;
+    const uint32_t t = (uint32_t)(i / per), c = (uint32_t)(i % per);
+    uint32_t lo = t * n, hi = lo + n;
+    if (c == per - 1) {
+        lo = hi;  // the table's end
+    } else {
+        const uint32_t k = (t << b) | c;
+        while (lo < hi) {
+            const uint32_t mid = lo + ((hi - lo) >> 1);
+            if (keys[mid] < k) lo = mid + 1; else hi = mid;
+        }
+    }
+    offs[i] = lo;
+}
+
 // Codes of every base row for all tables, stable radix sort of
-// (table << b | code, row), and the uint8 gather copy of the base.
+// (table << b | code, row), bucket starts, and the uint8 gather copy of the
+// base.
 int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag) {
     const uint32_t r_n = ix->n;
     const int L = ix->L, nbits = ix->nbits;
@@ -366,6 +408,13 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
             std::swap(ix->keys, ix->scratch_k);
             std::swap(ix->vals, ix->scratch_v);
         }
+        if (nbits <= OFFS_MAX_BITS && r_n > 0) {
+            const size_t no = (size_t)L * ((1u << nbits) + 1u);
+            if ((rc = reserve(h, ix->offs, no * 4)) != SIFT_OK) break;
+            bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
+                static_cast<const uint32_t*>(ix->keys.p), r_n, nbits, (uint32_t)L, static_cast<uint32_t*>(ix->offs.p));
+            if (hipGetLastError() != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "bucket offsets"); break; }
+        }
         if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "sync"); break; }
         release(ix->scratch_k);
         release(ix->scratch_v);
@@ -381,7 +430,7 @@ void sift_lsh_state_free(sift_context* h) {
     if (!h || !h->lsh_state) return;
     auto* s = static_cast<LshScratch*>(h->lsh_state);
     for (DevBuf* b : {&s->q_u8, &s->q_nrm, &s->qcodes, &s->flag, &s->q_tiles.tiles, &s->q_tiles.norm,
-                      &s->q_tiles.norm_lane, &s->fb_nc, &s->fb_list, &s->fb_count, &s->fb_map, &s->fb_out,
+                      &s->q_tiles.norm_lane, &s->fb_nc, &s->fb_dist, &s->fb_list, &s->fb_count, &s->fb_map, &s->fb_out,
                       &s->fb_q.tiles, &s->fb_q.norm, &s->fb_q.norm_lane})
         release(*b);
     delete s;
@@ -488,6 +537,7 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     a.keys = static_cast<const uint32_t*>(ix->keys.p);
     a.vals = static_cast<const uint32_t*>(ix->vals.p);
     a.qcodes = qcodes;
+    a.offs = ix->offs.p ? static_cast<const uint32_t*>(ix->offs.p) : nullptr;
     a.n = ix->n;
     a.q_n = q_n;
     a.L = ix->L;
@@ -547,10 +597,12 @@ int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_
 }
 
 int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_t bf, const void* q, int type,
-                            uint32_t q_n, uint32_t min_cand, uint32_t* idx, float* dist, uint32_t* ncand,
-                            uint32_t* matches, float ratio, uint32_t* n_fallback) {
+                            uint32_t q_n, uint32_t min_cand, float accept_ratio, uint32_t* idx, float* dist,
+                            uint32_t* ncand, uint32_t* matches, float ratio, uint32_t* n_fallback) {
     if (!h || !ix || !bf) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
     if (bf->r_n != ix->n) return set_error(h, SIFT_ERR_INVALID, "the brute-force index holds another base");
+    if (!(accept_ratio >= 0.f && accept_ratio <= 1.f))
+        return set_error(h, SIFT_ERR_INVALID, "accept_ratio must be in [0, 1] (0: count rule only)");
     if (n_fallback) *n_fallback = 0;
     if (q_n == 0) return SIFT_OK;
     if (!q) return set_error(h, SIFT_ERR_INVALID, "null queries");
@@ -561,14 +613,19 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     LshScratch* sc = scratch_of(h);
     SIFT_TRY(reserve(h, sc->fb_nc, (size_t)q_n * 4));
     uint32_t* nc = ncand ? ncand : static_cast<uint32_t*>(sc->fb_nc.p);
-    SIFT_TRY(sift_lsh_query(h, ix, q, type, q_n, idx, dist, nc, matches, ratio, 0));
-    // queries with fewer than min_cand candidates, in query order
+    float* dq = dist;  // the ratio rule reads the LSH distances
+    if (!dq && accept_ratio > 0.f) {
+        SIFT_TRY(reserve(h, sc->fb_dist, (size_t)q_n * 8));
+        dq = static_cast<float*>(sc->fb_dist.p);
+    }
+    SIFT_TRY(sift_lsh_query(h, ix, q, type, q_n, idx, dq, nc, matches, ratio, 0));
+    // queries the LSH answer does not settle
     SIFT_TRY(reserve(h, sc->fb_list, (size_t)q_n * 4 + 4));
     auto* list = static_cast<uint32_t*>(sc->fb_list.p);
     SIFT_TRY(reserve(h, sc->fb_count, 4));
     SIFT_HIP_TRY(h, hipMemsetAsync(sc->fb_count.p, 0, 4, h->stream));
-    select_low<<<(q_n + 255) / 256, 256, 0, h->stream>>>(nc, q_n, min_cand, static_cast<uint32_t*>(sc->fb_count.p),
-                                                         list);
+    select_low<<<(q_n + 255) / 256, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio,
+                                                         static_cast<uint32_t*>(sc->fb_count.p), list);
     SIFT_HIP_TRY(h, hipGetLastError());
     uint32_t cnt = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));

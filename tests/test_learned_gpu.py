@@ -179,9 +179,25 @@ def test_fallback_is_exact(handle):
     np.testing.assert_array_equal(dist[low], ed)
     np.testing.assert_array_equal(mat[low], em)
     np.testing.assert_array_equal(idx[~low], li[~low])
+    # ratio rule: also the queries whose best candidate fails
+    # sqrt(d1)/sqrt(d2) < 0.8 against the next one go to the exact path
+    say("fallback query, ratio rule")
+    idx, dist, nc, mat, nf = ix.query_fallback(q, bf, min_cand=32, accept_ratio=0.8)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amb = ~(np.sqrt(ld[:, 0]) / np.sqrt(ld[:, 1]) < np.float32(0.8))
+    sel = low | amb
+    assert nf == int(sel.sum()) and int(low.sum()) < nf < len(q)
+    ei, ed, _, em = knn2(handle, q[sel], b, compute=COMPUTE_BF16)
+    np.testing.assert_array_equal(idx[sel], ei)
+    np.testing.assert_array_equal(dist[sel], ed)
+    np.testing.assert_array_equal(mat[sel], em)
+    np.testing.assert_array_equal(idx[~sel], li[~sel])
+    np.testing.assert_array_equal(mat[~sel], lm[~sel])
     with pytest.raises(SiftError):
         other = BruteForceIndex(handle, b[:100], compute=COMPUTE_BF16)
         ix.query_fallback(q, other)
+    with pytest.raises(SiftError):
+        ix.query_fallback(q, bf, accept_ratio=1.5)
     ix.close()
     bf.close()
     mlp.close()
