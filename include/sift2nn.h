@@ -58,8 +58,14 @@ enum {
     SIFT_COMPUTE_F16 = 2,  /* FP16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F32 = 3,  /* FP32 MFMA                                      */
     SIFT_COMPUTE_I8 = 4,   /* int8 MFMA on (x-128), int32 accumulate         */
-    SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank (pairs plans) */
+    SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank, see below    */
 };
+/* SIFT_COMPUTE_FP8 (sift_knn2*, brute-force indexes, pairs plans): a coarse
+ * pass on per-row-scaled e4m3 keeps 16 candidate rows per query, and an exact
+ * int32 re-rank of those returns the two nearest among them with exact
+ * distances.  Needs integer data in [0,255] (SIFT_ERR_NOT_EXACT otherwise).
+ * Approximate: the true 2-NN can be missed (exact when every value is in
+ * [0,15]); no counterpart in the reference. */
 
 /* Hash kinds for the LSH index (lsh_type in the reference, src/lsh_h.cu:716-718). */
 enum {

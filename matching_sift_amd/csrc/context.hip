@@ -147,9 +147,9 @@ int sift_destroy(sift_handle_t h) {
     if (!h) return SIFT_OK;
     hipStreamSynchronize(h->stream);
     sift_lsh_state_free(h);
-    release(h->q_set.tiles); release(h->q_set.norm); release(h->q_set.norm_lane);
-    release(h->b_set.tiles); release(h->b_set.norm); release(h->b_set.norm_lane);
-    release(h->part); release(h->thresh); release(h->flag); release(h->stage_q); release(h->stage_r);
+    release_set(h->q_set);
+    release_set(h->b_set);
+    release(h->part); release(h->thresh); release(h->cand); release(h->flag); release(h->stage_q); release(h->stage_r);
     for (int i = 0; i < h->cap_pend; i++) {
         hipEventDestroy(h->pend[i].a);
         hipEventDestroy(h->pend[i].b);

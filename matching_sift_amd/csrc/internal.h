@@ -72,6 +72,7 @@ struct PackedSet {
     DevBuf tiles;  // [tiles][groups][64][16B]
     DevBuf norm;   // float[rows_pad] (int32 for KIND_I8), natural row order
     DevBuf norm_lane;  // [tiles][32] norms in accumulator-lane order (base only)
+    DevBuf u8, n2;     // KIND_FP8: uint8 rows [rows][128] and int32 |x|^2 for the exact re-rank
     uint32_t rows = 0, rows_pad = 0, ntiles = 0;
     int kind = 0;
 };
@@ -93,6 +94,18 @@ int pack_launch(sift_context* h, const void* in, int type, uint32_t n, const int
 // uint8 copy [n][128] and int32 squared norms of n input rows (flags non-integers).
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
           int* bad_dev);
+// pack_rows, plus for KIND_FP8 the uint8 copy and norms of the same rows.
+int pack_side(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
+              int role, PackedSet& o, int* bad_dev);
+void release_set(PackedSet& s);
+// Exact re-rank of 16 candidates per output (candidate row c of output o is
+// base row boff[o] + c; query row qsrc[o]; null qsrc / boff: o and 0):
+// int32 distances from the uint8 copies, the two smallest (d, row), and the
+// output formats.
+int fp8_rerank_launch(sift_context* h, const uint32_t* cand, const void* q_u8, const int* q_n2,
+                      const void* r_u8, const int* r_n2, const uint32_t* qsrc, const uint32_t* boff,
+                      uint32_t total, uint32_t* idx, float* dist, sift_float4* sorted, uint32_t* matches,
+                      float ratio);
 // Check that every value is an integer in [0,255]; result in *bad_dev (OR-ed).
 int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_dev);
 
@@ -147,6 +160,7 @@ struct sift_context {
     sift::PackedSet q_set, b_set;
     sift::DevBuf part;      // fused-kernel partial top-2
     sift::DevBuf thresh;    // per-query admission bounds from the seeding pass
+    sift::DevBuf cand;      // FP8 coarse pass: 16 candidate rows per query
     sift::DevBuf flag;      // int flags
     sift::DevBuf stage_q, stage_r;  // host-input staging
 

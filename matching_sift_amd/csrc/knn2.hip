@@ -36,6 +36,53 @@ struct MergeArgs {
 // In seeding mode the result is the query's shared bound: its 2nd-best key
 // over the rows merged.
 constexpr int MERGE_WAVES = 8;
+// FP8 coarse pass: relative widening of the seed bounds (coarse distances
+// differ from exact ones by a few percent for SIFT data).
+constexpr float FP8_SLACK = 1.f / 16.f;
+
+// FP8 coarse pass: the 16 smallest coarse (key, row) over every slab partial
+// of a query (two entries each), one wave per query: each lane keeps a sorted
+// top-16 of its own entries, then 16 rounds of a wave-wide minimum pop the
+// overall order.  Rows come out ascending by coarse key; absent: UINT32_MAX.
+__global__ __launch_bounds__(256) void merge_cand16(const uint4* __restrict__ part, uint32_t q_n,
+                                                    uint32_t q_stride, int n_slabs, uint32_t* __restrict__ cand) {
+    constexpr uint64_t NONE = ~0ull;
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= q_n) return;  // whole wave
+    uint64_t l[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) l[j] = NONE;
+    auto push = [&](uint32_t key_bits, uint32_t row) {
+        if (row == 0xFFFFFFFFu) return;
+        const int o = Key<KIND_FP8>::ord(__uint_as_float(key_bits));
+        uint64_t x = ((uint64_t)((uint32_t)o ^ 0x80000000u) << 32) | row;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {  // insertion: l stays ascending
+            const uint64_t lo = min(l[j], x);
+            x = max(l[j], x);
+            l[j] = lo;
+        }
+    };
+    for (int s = lane; s < n_slabs; s += 64) {
+        const uint4 p = part[(size_t)s * q_stride + q];
+        push(p.x, p.y);
+        push(p.z, p.w);
+    }
+    uint32_t out = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint64_t m = l[0];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) m = min(m, __shfl_xor(m, off));
+        if (lane == r) out = m == NONE ? 0xFFFFFFFFu : (uint32_t)m;
+        const bool pop = l[0] == m && m != NONE;  // rows are unique: one lane holds m
+#pragma unroll
+        for (int j = 0; j < 15; j++) l[j] = pop ? l[j + 1] : l[j];
+        l[15] = pop ? NONE : l[15];
+    }
+    if (lane < 16) cand[(size_t)q * 16 + lane] = out;
+}
 template <int KIND>
 __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
@@ -62,7 +109,13 @@ __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
     }
     const bool has1 = i1 < a.r_n, has2 = i2 < a.r_n;
     if (a.bound_out) {  // inclusive bound: the 2nd-best key over the rows merged
-        a.bound_out[q] = Key<KIND>::ord(has2 ? m2 : Key<KIND>::none());
+        key_t b = has2 ? m2 : Key<KIND>::none();
+        if constexpr (KIND == KIND_FP8) {
+            // coarse keys: widen by FP8_SLACK of the coarse distance so rows
+            // whose e4m3 rounding pushed them past the bound still enter
+            if (has2) b = b + FP8_SLACK * fabsf(b + a.q_norm[q]) + 1.f;
+        }
+        a.bound_out[q] = Key<KIND>::ord(b);
         return;
     }
     float d1 = FLT_MAX, d2 = FLT_MAX;
@@ -220,6 +273,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
             case KIND_BF16: merge_top2<KIND_BF16><<<g, b, 0, h->stream>>>(ma); break;
             case KIND_F16: merge_top2<KIND_F16><<<g, b, 0, h->stream>>>(ma); break;
             case KIND_F32: merge_top2<KIND_F32><<<g, b, 0, h->stream>>>(ma); break;
+            case KIND_FP8: merge_top2<KIND_FP8><<<g, b, 0, h->stream>>>(ma); break;
             default: merge_top2<KIND_I8><<<g, b, 0, h->stream>>>(ma); break;
         }
     };
@@ -235,6 +289,17 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         ProfScope ps(h, PROF_FUSED);
         sh.fn(mk_fused(l), (unsigned)(slabs[l] * n_groups), h->stream);
         SIFT_HIP_TRY(h, hipGetLastError());
+    }
+    if (bs.kind == KIND_FP8) {  // 16 coarse candidates per query, exact re-rank
+        if (!qs.u8.p || !bs.u8.p) return set_error(h, SIFT_ERR_INVALID, "FP8 sets need their uint8 copies");
+        ProfScope ps(h, PROF_FP8);
+        SIFT_TRY(reserve(h, h->cand, (size_t)q_n * 16 * 4));
+        auto* cand = static_cast<uint32_t*>(h->cand.p);
+        merge_cand16<<<(q_n + 3) / 4, 256, 0, h->stream>>>(part, q_n, q_stride, p0[n_launch], cand);
+        SIFT_HIP_TRY(h, hipGetLastError());
+        return fp8_rerank_launch(h, cand, qs.u8.p, static_cast<const int*>(qs.n2.p), bs.u8.p,
+                                 static_cast<const int*>(bs.n2.p), nullptr, nullptr, q_n, out.idx, out.dist,
+                                 out.sorted, out.matches, out.ratio);
     }
     {
         ProfScope ps(h, PROF_MERGE);
@@ -317,6 +382,7 @@ int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32
         case SIFT_COMPUTE_F16: *kind = KIND_F16; return SIFT_OK;
         case SIFT_COMPUTE_F32: *kind = KIND_F32; return SIFT_OK;
         case SIFT_COMPUTE_I8: *kind = KIND_I8; return SIFT_OK;
+        case SIFT_COMPUTE_FP8: *kind = KIND_FP8; return SIFT_OK;
         case SIFT_COMPUTE_AUTO: break;
         default: return set_error(h, SIFT_ERR_INVALID, "unknown compute kind " + std::to_string(compute));
     }
@@ -357,15 +423,16 @@ int knn2_impl(sift_context* h, const void* q, const void* r, int type, uint32_t 
     int* flag = static_cast<int*>(h->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 4, h->stream));
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    SIFT_TRY(pack_rows(h, r, type, r_n, rows_b, kind, PACK_BASE, h->b_set, flag));
-    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(kind, q_n), kind, PACK_QUERY, h->q_set, flag));
+    SIFT_TRY(pack_side(h, r, type, r_n, rows_b, kind, PACK_BASE, h->b_set, flag));
+    SIFT_TRY(pack_side(h, q, type, q_n, query_rows_pad(kind, q_n), kind, PACK_QUERY, h->q_set, flag));
     SIFT_TRY(knn2_packed(h, h->q_set, h->b_set, q_n, r_n, out));
     if (blocking) {
         int bad = 0;
         SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
         SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-        if (bad && kind == KIND_I8)
-            return set_error(h, SIFT_ERR_NOT_EXACT, "SIFT_COMPUTE_I8 needs integer data in [0,255]");
+        if (bad && (kind == KIND_I8 || kind == KIND_FP8))
+            return set_error(h, SIFT_ERR_NOT_EXACT, kind == KIND_I8 ? "SIFT_COMPUTE_I8 needs integer data in [0,255]"
+                                                                    : "SIFT_COMPUTE_FP8 needs integer data in [0,255]");
     }
     return SIFT_OK;
 }
@@ -529,13 +596,18 @@ int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, 
     int* flag = static_cast<int*>(h->flag.p);
     (void)hipMemsetAsync(flag, 0, 4, h->stream);
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    int rc = pack_rows(h, r, type, r_n, rows_b, kind, PACK_BASE, ix->base, flag);
+    int rc = pack_side(h, r, type, r_n, rows_b, kind, PACK_BASE, ix->base, flag);
     if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess)
         rc = set_error(h, SIFT_ERR_HIP, "index build sync");
+    if (rc == SIFT_OK && (kind == KIND_I8 || kind == KIND_FP8)) {
+        int bad = 0;
+        if (hipMemcpy(&bad, flag, 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = set_error(h, SIFT_ERR_HIP, "index build flag");
+        else if (bad)
+            rc = set_error(h, SIFT_ERR_NOT_EXACT, "I8 and FP8 indexes need integer data in [0,255]");
+    }
     if (rc != SIFT_OK) {
-        release(ix->base.tiles);
-        release(ix->base.norm);
-        release(ix->base.norm_lane);
+        release_set(ix->base);
         delete ix;
         return rc;
     }
@@ -552,19 +624,24 @@ int sift_bf_index_query(sift_handle_t h, sift_bf_index_t ix, const void* q, int 
     if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
     SIFT_TRY(reserve(h, h->flag, 16));
     int* flag = static_cast<int*>(h->flag.p);
-    SIFT_TRY(pack_rows(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, h->q_set, flag));
+    SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 4, h->stream));
+    SIFT_TRY(pack_side(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, h->q_set, flag));
     Knn2Out o{idx, dist, sorted, matches, ratio};
     SIFT_TRY(knn2_packed(h, h->q_set, ix->base, q_n, ix->r_n, o));
-    if (blocking) SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (blocking) {
+        int bad = 0;
+        SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
+        SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+        if (bad && (ix->kind == KIND_I8 || ix->kind == KIND_FP8))
+            return set_error(h, SIFT_ERR_NOT_EXACT, "I8 and FP8 indexes need integer queries in [0,255]");
+    }
     return SIFT_OK;
 }
 
 int sift_bf_index_destroy(sift_bf_index_t ix) {
     if (!ix) return SIFT_OK;
     (void)hipStreamSynchronize(ix->h->stream);
-    release(ix->base.tiles);
-    release(ix->base.norm);
-    release(ix->base.norm_lane);
+    release_set(ix->base);
     delete ix;
     return SIFT_OK;
 }

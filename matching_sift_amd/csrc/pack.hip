@@ -429,6 +429,17 @@ int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_
     return SIFT_OK;
 }
 
+int pack_side(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
+              int role, PackedSet& o, int* bad_dev) {
+    SIFT_TRY(pack_rows(h, in, type, n, rows_pad, kind, role, o, bad_dev));
+    if (kind == KIND_FP8) SIFT_TRY(to_u8(h, in, type, n, o.u8, o.n2, bad_dev));
+    return SIFT_OK;
+}
+
+void release_set(PackedSet& s) {
+    for (DevBuf* b : {&s.tiles, &s.norm, &s.norm_lane, &s.u8, &s.n2}) release(*b);
+}
+
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
           int* bad_dev) {
     SIFT_TRY(reserve(h, out, (size_t)n * 128));

@@ -74,6 +74,7 @@ __global__ __launch_bounds__(256) void fp8_rerank(const uint32_t* __restrict__ c
                                                   const uint32_t* __restrict__ qsrc,
                                                   const uint32_t* __restrict__ boff, uint32_t total,
                                                   uint32_t* __restrict__ idx, float* __restrict__ dist,
+                                                  sift_float4* __restrict__ sorted,
                                                   uint32_t* __restrict__ matches, float ratio) {
     const uint64_t NONE = ~0ull;
     const uint32_t gid = blockIdx.x * 256 + threadIdx.x;
@@ -83,8 +84,8 @@ __global__ __launch_bounds__(256) void fp8_rerank(const uint32_t* __restrict__ c
     const uint32_t c = cand[(size_t)o * 16 + s];
     uint64_t m1 = NONE, m2 = NONE;
     if (c != 0xFFFFFFFFu) {
-        const uint32_t qr = qsrc[o];
-        const size_t br = (size_t)boff[o] + c;
+        const uint32_t qr = qsrc ? qsrc[o] : o;
+        const size_t br = (size_t)(boff ? boff[o] : 0u) + c;
         const uint4* qp = q_u8 + (size_t)qr * 8;
         const uint4* bp = r_u8 + br * 8;
         uint32_t dot = 0;
@@ -118,6 +119,14 @@ __global__ __launch_bounds__(256) void fp8_rerank(const uint32_t* __restrict__ c
         if (dist) {
             dist[2 * (size_t)o] = d1;
             dist[2 * (size_t)o + 1] = d2;
+        }
+        if (sorted) {
+            sift_float4 f;
+            f.x = d1;
+            f.y = d2;
+            f.z = h1 ? (float)j1 : 3.402823466e+38f;
+            f.w = h2 ? (float)j2 : 3.402823466e+38f;
+            sorted[o] = f;
         }
         if (matches) matches[o] = (h1 && sqrtf(d1) / sqrtf(d2) < ratio) ? j1 : 0xFFFFFFFFu;
     }
@@ -242,7 +251,7 @@ int run_sequence(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx
             static_cast<const uint32_t*>(p->cand.p), static_cast<const uint4*>(p->q_u8.p),
             static_cast<const int*>(p->q_n2.p), static_cast<const uint4*>(p->r_u8.p),
             static_cast<const int*>(p->r_n2.p), static_cast<const uint32_t*>(p->qsrc.p),
-            static_cast<const uint32_t*>(p->boff.p), (uint32_t)p->total_q, idx, dist, matches, ratio);
+            static_cast<const uint32_t*>(p->boff.p), (uint32_t)p->total_q, idx, dist, nullptr, matches, ratio);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     return SIFT_OK;
@@ -461,6 +470,21 @@ int kind_of(sift_context* h, int compute, int* kind) {
 bool out_ok(const void* p) { return !p || device_accessible(p); }
 
 }  // namespace
+
+namespace sift {
+int fp8_rerank_launch(sift_context* h, const uint32_t* cand, const void* q_u8, const int* q_n2,
+                      const void* r_u8, const int* r_n2, const uint32_t* qsrc, const uint32_t* boff,
+                      uint32_t total, uint32_t* idx, float* dist, sift_float4* sorted, uint32_t* matches,
+                      float ratio) {
+    if (total == 0) return SIFT_OK;
+    const unsigned grid = (unsigned)(((size_t)total * 16 + 255) / 256);
+    fp8_rerank<<<grid, 256, 0, h->stream>>>(cand, static_cast<const uint4*>(q_u8), q_n2,
+                                            static_cast<const uint4*>(r_u8), r_n2, qsrc, boff, total, idx,
+                                            dist, sorted, matches, ratio);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+}  // namespace sift
 
 extern "C" {
 

@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from matching_sift_amd import (BF16_DEVICE, COMPUTE_BF16, COMPUTE_F16, COMPUTE_F32, COMPUTE_I8,
-                               COMPUTE_AUTO, FLOAT_DEVICE, FLOAT_HOST, HALF_DEVICE, U8_DEVICE,
+                               COMPUTE_AUTO, COMPUTE_FP8, FLOAT_DEVICE, FLOAT_HOST, HALF_DEVICE, U8_DEVICE,
                                DeviceArray, BruteForceIndex, SiftError, knn2, datagen)
 from matching_sift_amd import api
 
@@ -242,3 +242,58 @@ def test_full_size_default_chain(handle):
     idx, dist, srt, mat = knn2(handle, q, b, compute=COMPUTE_BF16, ratio=0.8)
     np.testing.assert_array_equal(srt, s)
     np.testing.assert_array_equal(mat, mt)
+
+
+@pytest.mark.parametrize("shape", ["2,4,2", "2,8,2", "4,4,2"])
+def test_fp8_exact_domain(handle, shape, monkeypatch):
+    """FP8 coarse pass + exact re-rank.  With every value in [0,15] each
+    per-row-scaled e4m3 operand is exact, so the coarse keys equal the exact
+    keys and the result is the oracle's, bit for bit, through the seeded chain,
+    with duplicates and exact ties; also through a brute-force index."""
+    monkeypatch.setenv("SIFT_SHAPE", shape)
+    rng = np.random.default_rng(21)
+    r = rng.integers(0, 16, size=(40_000, 128)).astype(np.float32)
+    r[30_000:30_050] = r[100:150]
+    q = rng.integers(0, 16, size=(300, 128)).astype(np.float32)
+    q[:50] = r[100:150]
+    say("fp8 exact domain", shape)
+    s, mt = expected(q, r)
+    idx, dist, srt, mat = knn2(handle, q, r, compute=COMPUTE_FP8, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+    ix = BruteForceIndex(handle, r, compute=COMPUTE_FP8)
+    gi, gd = to_idx_dist(s)
+    ii, dd = ix.query(q)
+    np.testing.assert_array_equal(ii, gi)
+    np.testing.assert_array_equal(dd, gd)
+    ix.close()
+
+
+def test_fp8_sift_recall(handle):
+    """On SIFT-like data the FP8 pass is approximate: the first neighbour is
+    found for at least 99% of the queries, the ratio-test output agrees for
+    at least 99%, and every returned distance is the exact squared L2 of the
+    returned row (the re-rank is exact)."""
+    b = datagen.base(40_000, seed=95)
+    q = datagen.queries(b, 500, seed=96)
+    say("fp8 sift recall")
+    s, mt = expected(q, b)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=COMPUTE_FP8, ratio=0.8)
+    gi, _ = to_idx_dist(s)
+    nn1 = float((idx[:, 0] == gi[:, 0]).mean())
+    agree = float((mat == mt).mean())
+    say("fp8 recall nn1 %.4f, 2-nn %.4f, match %.4f" % (nn1, float((idx == gi).all(1).mean()), agree))
+    assert nn1 >= 0.99 and agree >= 0.99
+    assert (idx != 0xFFFFFFFF).all()
+    qi, bi = q.astype(np.int64), b.astype(np.int64)
+    for j in (0, 1):
+        d = ((qi - bi[idx[:, j]]) ** 2).sum(1)
+        np.testing.assert_array_equal(dist[:, j], d.astype(np.float32))
+    assert (dist[:, 0] <= dist[:, 1]).all()
+
+
+def test_fp8_rejects_non_integer(handle):
+    q = np.full((2, 128), 0.5, np.float32)
+    with pytest.raises(SiftError) as e:
+        knn2(handle, q, q, compute=COMPUTE_FP8)
+    assert e.value.code == -5
