@@ -128,6 +128,9 @@ struct SlabTask {
     int slab;            // !PAIRS: partial-result slab
 };
 
+template <typename T>
+__device__ inline T min3k(T a, T b, T c) { return min(min(a, b), c); }
+
 template <int KIND, int QW, int WAVES, int TPS, bool PAIRS>
 __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& task) {
     using KTk = KT<KIND>;
@@ -373,11 +376,15 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             fmask[tt] = 0;
 #pragma unroll
             for (int qb = 0; qb < QW; qb++) {
-                bool f = last;
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-                    f |= min(min(tv[qb][4 * c], tv[qb][4 * c + 1]), min(tv[qb][4 * c + 2], tv[qb][4 * c + 3])) < thr[qb];
-                fmask[tt] |= (__builtin_amdgcn_ballot_w64(f) != 0 ? 1u : 0u) << qb;
+                // minimum of the 16 keys in 8 min/min3 ops, one compare, and
+                // the wave-wide test on the scalar side
+                const key_t* v = tv[qb];
+                const key_t a0 = min3k(v[0], v[1], v[2]), a1 = min3k(v[3], v[4], v[5]);
+                const key_t a2 = min3k(v[6], v[7], v[8]), a3 = min3k(v[9], v[10], v[11]);
+                const key_t a4 = min3k(v[12], v[13], v[14]);
+                const key_t mn = min(min3k(a0, a1, a2), min3k(a3, a4, v[15]));
+                const bool hit = __builtin_amdgcn_ballot_w64(mn < thr[qb]) != 0;
+                fmask[tt] |= ((hit || last) ? 1u : 0u) << qb;
             }
             if (tt >= ntile) fmask[tt] = 0;
             any_flag |= fmask[tt] != 0;
