@@ -175,7 +175,7 @@ struct FusedArgs {
     float* out_dist;
     uint32_t* out_matches;
     float ratio;
-    int neg64;  // -64, passed at run time so the I8 key is one v_mad_i32_i24
+    int neg64;  // -64, passed at run time so I8 keys take one v_mad_i32_i24 (brute force: -64 / 32)
     uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
 };
 

@@ -154,6 +154,9 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     __shared__ __align__(16) float s_nl[3][WAVES][NLC * 64];
 
     const int t_begin = task.t_begin, t_end = task.t_end;
+    // I8 keys |b'|^2 - 2 q'.b': |q'.b'| < 2^21, so one 24-bit multiply-add;
+    // the factor comes in at run time (a literal -2 becomes shift + subtract)
+    const int neg2 = a.neg64 / 32;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6, h = lane >> 5;
     const int qblk0 = task.qblk0 + wave * QW;  // this wave's first query block
@@ -310,7 +313,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                     if constexpr (FLOAT_KEY)
                         tv[qb][j] = acc[qb][j];
                     else
-                        tv[qb][j] = nlr[tt][j] - 2 * acc[qb][j];
+                        tv[qb][j] = __mul24((int)acc[qb][j], neg2) + nlr[tt][j];  // one v_mad_i32_i24
                 }
         };
 
@@ -414,7 +417,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                             if constexpr (FLOAT_KEY)
                                 tv[j] = acc[j];
                             else
-                                tv[j] = nlr[tt][j] - 2 * acc[j];
+                                tv[j] = __mul24((int)acc[j], neg2) + nlr[tt][j];
                         }
                     }
                     const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;

@@ -249,6 +249,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         fa.tiles_per_slab = tps_slab[l];
         fa.n_slabs = slabs[l];
         fa.n_groups = n_groups;
+        fa.neg64 = -64;
         return fa;
     };
     auto mk_merge = [&](int n_part) {
