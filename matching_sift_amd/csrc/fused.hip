@@ -77,6 +77,44 @@ __device__ inline void dma4(const void* g, const void* lds) {
     (void)lds;
 #endif
 }
+// The same copies with a uniform base address in SGPRs plus a per-lane 32-bit
+// offset (the saddr form): no 64-bit address arithmetic per lane.
+__device__ inline void dma16_s(const void* sbase, uint32_t voff, uint32_t lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t save;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dwordx4 %2, %3\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(save)
+        : "s"(lds), "v"(voff), "s"(sbase)
+        : "memory");
+#else
+    (void)sbase;
+    (void)voff;
+    (void)lds;
+#endif
+}
+__device__ inline void dma4_s(const void* sbase, uint32_t voff, uint32_t lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t save;
+    asm volatile(
+        "s_mov_b32 %0, m0\n\t"
+        "s_mov_b32 m0, %1\n\t"
+        "s_nop 0\n\t"
+        "global_load_lds_dword %2, %3\n\t"
+        "s_mov_b32 m0, %0"
+        : "=&s"(save)
+        : "s"(lds), "v"(voff), "s"(sbase)
+        : "memory");
+#else
+    (void)sbase;
+    (void)voff;
+    (void)lds;
+#endif
+}
 // s_waitcnt vmcnt(N) lgkmcnt(0) (gfx9 encoding: vmcnt[3:0] and [15:14],
 // expcnt[6:4] left at its maximum, lgkmcnt[11:8]).
 template <int N>
@@ -158,7 +196,8 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     // the factor comes in at run time (a literal -2 becomes shift + subtract)
     const int neg2 = a.neg64 / 32;
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, h = lane >> 5;
+    const int lane = tid & 63, h = lane >> 5;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar address arithmetic
     const int qblk0 = task.qblk0 + wave * QW;  // this wave's first query block
 
     uint4 qop[QW][G];
@@ -201,14 +240,26 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         const bool full = nt == TPS;
         // chunk c = tile c / G, group c % G (the packed norm group is skipped)
         const uint4* src = a.b_tiles + (size_t)ts * TG * 64;
-        auto gsrc = [&](int c) { return src + (size_t)((c / G) * TG + c % G) * 64 + lane; };
+        auto gsrc = [&](int c) { return src + (size_t)((c / G) * TG + c % G) * 64; };  // chunk base (uniform)
+        auto lds_of = [&](const void* p) { return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)p); };
+        if (full) {  // the common case: every copy, no per-copy branch
 #pragma unroll
-        for (int i = 0; i < NPW; i++) {
-            const int c = wave + i * WAVES;
-            if (full)  // padding slots reload chunk 0 (never read back)
-                dma16(gsrc(c < CH ? c : 0), &s_tiles[bsel][c * 64]);
-            else if (c < nt * G)
-                dma16(gsrc(c), &s_tiles[bsel][c * 64]);
+            for (int i = 0; i < NPW; i++) {
+                const int c = wave + i * WAVES;  // padding slots reload chunk 0 (never read back)
+                dma16_s(gsrc(c < CH ? c : 0), (uint32_t)lane * 16u, lds_of(&s_tiles[bsel][c * 64]));
+            }
+            if constexpr (NLC * 64 <= TPS * 32) {  // every norm copy in range
+#pragma unroll
+                for (int c = 0; c < NLC; c++)
+                    dma4_s(a.b_nl + (size_t)ts * 32 + c * 64, (uint32_t)lane * 4u, lds_of(&s_nl[bsel][wave][c * 64]));
+                return true;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NPW; i++) {
+                const int c = wave + i * WAVES;
+                if (c < nt * G) dma16_s(gsrc(c), (uint32_t)lane * 16u, lds_of(&s_tiles[bsel][c * 64]));
+            }
         }
 #pragma unroll
         for (int c = 0; c < NLC; c++)
