@@ -498,8 +498,11 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                             thr[qb] = min(thr[qb], m2[qb]);
                         }
                     }
-                    // a real 2nd best found here bounds the query everywhere
-                    if (a.bound && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
+                    // a real 2nd best found here bounds the query everywhere (not
+                    // for FP8: its coarse keys may rank the true 2-NN above a
+                    // coarse 2nd best, so only the widened seed bounds apply,
+                    // and the candidates do not depend on block timing)
+                    if (KIND != KIND_FP8 && a.bound && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
                         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
                         if (q - task.q_row0 < task.q_count) atomicMin(&a.bound[q], Key<KIND>::ord(m2[qb]));
                     }
@@ -685,9 +688,9 @@ const FusedShape kShapes[] = {
     SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F16, 2, 4, 2),
     SHAPE(KIND_F32, 2, 4, 1),
+    SHAPE(KIND_FP8, 4, 4, 2),
     SHAPE(KIND_FP8, 2, 4, 2),
     SHAPE(KIND_FP8, 2, 8, 2),
-    SHAPE(KIND_FP8, 4, 4, 2),
     SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 4, 4, 2),
     SHAPE(KIND_I8, 2, 8, 2),
