@@ -303,6 +303,38 @@ def learned_bench(args, ms, h):
     bf.close()
 
 
+def sift1m_bench(args, ms):
+    """Real ANN_SIFT1M files when present (sift_base.fvecs, sift_query.fvecs,
+    sift_groundtruth.ivecs under --data; raw integer values): exact BF16 and
+    FP8 2-NN of all queries, queries/sec, and the reference's recall counts
+    (src/test.cu:224-301) of the first neighbour and of the ratio-test
+    matches against the ground-truth file."""
+    names = ("sift_base.fvecs", "sift_query.fvecs", "sift_groundtruth.ivecs")
+    files = [os.path.join(args.data, n) for n in names]
+    missing = [f for f in files if not os.path.exists(f)]
+    if missing:
+        print(json.dumps({"workload": "sift1m", "skipped": "not found: %s" % ", ".join(missing)}), flush=True)
+        return
+    b, q, gt = ms.read_vecs(files[0]), ms.read_vecs(files[1]), ms.read_vecs(files[2])
+    M, N = q.shape[0], b.shape[0]
+    h = ms.Handle(seed=1)
+    qd, bd = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b)
+    out = {"workload": "sift1m", "n_base": N, "n_query": M, "gt_k": int(gt.shape[1])}
+    for name, comp in (("bf16", ms.COMPUTE_BF16), ("fp8", ms.COMPUTE_FP8)):
+        r = ms.knn2(h, qd, bd, compute=comp, ratio=0.8)  # first call: shape timing
+        h.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = ms.knn2(h, qd, bd, compute=comp, ratio=0.8)
+        qps = M * args.steps / (time.perf_counter() - t0)
+        out[name] = {"queries_per_s": qps,
+                     "recall_nn1": ms.recall_count(r[0][:, 0], gt),
+                     "recall_matches": ms.recall_count(r[3], gt),
+                     "matches": int((r[3] != 0xFFFFFFFF).sum()),
+                     "nn1_equals_gt_first": float((r[0][:, 0] == gt[:, 0]).mean())}
+    print(json.dumps(out), flush=True)
+
+
 def copy_calibration(args):
     """A plain device copy of known size (torch's elementwise copy kernel):
     the reference point that scales the FETCH_SIZE / WRITE_SIZE counters of a
@@ -330,7 +362,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
-    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy", "learned"])
+    ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy", "learned", "sift1m"])
+    ap.add_argument("--data", default=os.path.join(ROOT, "data", "sift"), help="ANN_SIFT1M files (--workload sift1m)")
     ap.add_argument("--learned-bits", default="16,20,24")
     ap.add_argument("--learned-tables", default="1,2,4,8,16")
     ap.add_argument("--train-n", type=int, default=200_000)
@@ -362,6 +395,9 @@ def main():
         return
     if args.workload == "learned":
         learned_bench(args, ms, ms.Handle(seed=1))
+        return
+    if args.workload == "sift1m":
+        sift1m_bench(args, ms)
         return
     log("generating data N=%d M=%d" % (args.n, args.m))
     b = datagen.base(args.n)

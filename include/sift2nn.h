@@ -310,6 +310,25 @@ int sift_match_pairs(sift_handle_t h, const void* q_desc, const void* r_desc, in
                      const uint32_t* r_off, uint32_t n_pairs, const uint32_t* pairs,
                      int compute, uint32_t* idx, float* dist, uint32_t* matches, float ratio);
 
+/* ---- SIFT vector files and recall (reference src/test.cu:76-155, :224-301) ----
+ * Records are an int32 dimension d followed by d elements: float32 (.fvecs),
+ * int32 (.ivecs: ground-truth neighbour ids) or uint8 (.bvecs).  Host only;
+ * errors go to sift_last_error(NULL). */
+enum { SIFT_VECS_F32 = 0, SIFT_VECS_I32 = 1, SIFT_VECS_U8 = 2 };
+/* Number of records and their dimension (checked: whole records). */
+int sift_vecs_info(const char* path, int elem, uint32_t* n, uint32_t* dim);
+/* Records [first, first + n) into out, n x cols row-major (cols 0 = d; the
+ * first cols elements of each record, as the reference reads the first
+ * `recall` of 100 ground-truth ids).  F32/U8 rows become float32 times
+ * scale (the reference's real-data mode divides by 512); I32 rows stay int32. */
+int sift_vecs_read(const char* path, int elem, uint64_t first, uint32_t n, uint32_t cols, float scale,
+                   void* out);
+/* hits[k] = number of queries whose found[q] is among the first depths[k]
+ * ids of gt[q][0..gt_k) (UINT32_MAX never counts), 1 <= depths[k] <= gt_k:
+ * the reference's recall loop, as counts of found rather than missed. */
+int sift_recall_count(const uint32_t* found, uint32_t q_n, const int32_t* gt, uint32_t gt_k,
+                      const uint32_t* depths, int n_depths, uint32_t* hits);
+
 #ifdef __cplusplus
 }
 #endif

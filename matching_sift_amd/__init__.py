@@ -15,5 +15,5 @@ from ._lib import (  # noqa: F401
 from .api import (  # noqa: F401
     DeviceArray, Handle, to_device, knn2, cublas_2nn_sift, device_brute, cublas_2nn_f,
     BruteForceIndex, LSHIndex, lsh_gpu, PairsPlan, match_pairs, PAIRS_GRAPH, PAIRS_AUTOTUNE,
-    HashMLP, TrainOpts, MLP_NPARAM,
+    HashMLP, TrainOpts, MLP_NPARAM, vecs_info, read_vecs, recall_count,
 )

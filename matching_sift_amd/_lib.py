@@ -26,6 +26,9 @@ vp, u32, u64, i32, f32, sz = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.c_fl
 # name -> (restype, argtypes); every function of include/sift2nn.h
 SIGNATURES = {
     "sift_create": (i32, [C.POINTER(vp)]),
+    "sift_vecs_info": (i32, [C.c_char_p, i32, C.POINTER(u32), C.POINTER(u32)]),
+    "sift_vecs_read": (i32, [C.c_char_p, i32, u64, u32, u32, f32, vp]),
+    "sift_recall_count": (i32, [vp, u32, vp, u32, vp, i32, vp]),
     "sift_destroy": (i32, [vp]),
     "sift_last_error": (C.c_char_p, [vp]),
     "sift_last_status": (i32, []),

@@ -444,3 +444,49 @@ def lsh_gpu(q_points, r_points, type_, q_n, r_n, matches, threshold, handle, str
     """src/lsh_h.cu:722 — build l tables of nbits bits, query, ratio-test."""
     return check(lib().lsh_gpu(q_points.ptr, r_points.ptr, type_, q_n, r_n, matches.ptr,
                                float(threshold), handle.h, stream_n, l, lsh_type, nbits), handle.h)
+
+
+# --------------------------------------------------------- SIFT files ----
+VECS_KIND = {"f": 0, "i": 1, "b": 2}  # .fvecs, .ivecs, .bvecs
+
+
+def _vecs_kind(path, kind):
+    if kind is None:
+        kind = {".fvecs": "f", ".ivecs": "i", ".bvecs": "b"}.get(path[-6:], None)
+        if kind is None:
+            raise ValueError("cannot tell the element kind of %s (give kind='f', 'i' or 'b')" % path)
+    return VECS_KIND[kind]
+
+
+def vecs_info(path, kind=None):
+    """(records, dimension) of a .fvecs / .ivecs / .bvecs file."""
+    n, d = C.c_uint32(), C.c_uint32()
+    check(lib().sift_vecs_info(path.encode(), _vecs_kind(path, kind), C.byref(n), C.byref(d)), None)
+    return n.value, d.value
+
+
+def read_vecs(path, n=None, first=0, cols=None, scale=1.0, kind=None):
+    """Records [first, first + n) of a SIFT vector file as a numpy array:
+    float32 for .fvecs / .bvecs (times scale), int32 for .ivecs; cols keeps
+    the first cols elements of each record (src/test.cu:76-155)."""
+    k = _vecs_kind(path, kind)
+    total, d = vecs_info(path, kind)
+    n = total - first if n is None else n
+    cols = d if cols is None else cols
+    out = np.empty((n, cols), np.int32 if k == 1 else np.float32)
+    check(lib().sift_vecs_read(path.encode(), k, first, n, cols, float(scale), out.ctypes.data), None)
+    return out
+
+
+def recall_count(found, gt, depths=(1, 10, 20, 30, 40, 50, 60, 70, 80, 90, 100)):
+    """Per depth R: how many queries have their returned match among the first
+    R ground-truth ids (the reference's recall loop, src/test.cu:224-301)."""
+    found = np.ascontiguousarray(found, np.uint32)
+    gt = np.ascontiguousarray(gt, np.int32)
+    dep = np.ascontiguousarray(depths, np.uint32)
+    hits = np.zeros(len(dep), np.uint32)
+    if gt.ndim != 2 or gt.shape[0] != found.shape[0]:
+        raise ValueError("gt must be [queries][k]")
+    check(lib().sift_recall_count(found.ctypes.data, found.shape[0], gt.ctypes.data, gt.shape[1], dep.ctypes.data,
+                                  len(dep), hits.ctypes.data), None)
+    return dict(zip(dep.tolist(), hits.tolist()))
