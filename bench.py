@@ -16,6 +16,8 @@ Also reported on the same line:
   cpu_baseline : the oracle's host_brute restatement (oracle/) on the box's
                  host cores over a bounded query sample against the full base
   index_qps    : queries/sec with the base packed once (sift_bf_index_*)
+  host_input_qps: queries/sec with both sets in page-locked host memory
+                 (SIFT_FLOAT_HOST: the call copies them to the device)
 
     python bench.py [--steps K] [--warmup W] [--compute bf16|f16|f32|i8]
 """
@@ -361,6 +363,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=32, help="CPU baseline batch size")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-input (H2D included) rate")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "fused_traffic.json"))
     ap.add_argument("--workload", default="brute", choices=["brute", "lsh", "pairs", "copy", "learned", "sift1m"])
     ap.add_argument("--data", default=os.path.join(ROOT, "data", "sift"), help="ANN_SIFT1M files (--workload sift1m)")
@@ -469,6 +472,35 @@ def main():
     index_qps = M * args.steps / (time.perf_counter() - t1)
     ix.close()
 
+    # host-resident inputs (SIFT_FLOAT_HOST in page-locked memory): the call
+    # copies both sets to the device first; reported beside `value`, not as it
+    host_qps, host_ms = None, None
+    if not args.no_host:
+        import ctypes as C
+        bufs = []
+        try:
+            for arr in (q, b):
+                ptr = C.c_void_p()
+                ms._lib.check(L.sift_host_alloc(C.byref(ptr), arr.nbytes))
+                bufs.append(ptr)
+                np.ctypeslib.as_array((C.c_float * arr.size).from_address(ptr.value))[:] = arr.ravel()
+            hout = tuple(o.ptr if o is not None else None for o in outs)
+
+            def host_step():
+                rc = L.sift_knn2(h.h, bufs[0], bufs[1], ms.FLOAT_HOST, M, N, compute, *hout, 0.8)
+                if rc != 0:
+                    raise ms.SiftError(rc, L.sift_last_error(h.h).decode())
+            host_step()
+            n_host = max(2, min(args.steps, 5))
+            t2 = time.perf_counter()
+            for _ in range(n_host):
+                host_step()
+            host_ms = (time.perf_counter() - t2) / n_host * 1e3
+            host_qps = M / host_ms * 1e3
+        finally:
+            for ptr in bufs:
+                L.sift_host_free(ptr)
+
     traffic = None
     if os.path.exists(args.traffic_json):
         try:
@@ -513,6 +545,8 @@ def main():
                      "launches_per_step": fused_n / args.steps, "flops_per_step": flops},
         "cpu_baseline": cpu,
         "index_qps": index_qps,
+        "host_input_qps": host_qps,
+        "host_input_ms_per_step": host_ms,
         "breakdown_ms_per_step": {"pack": pack_ms / args.steps, "seed": seed_ms / args.steps,
                                   "fused": fused_ms / args.steps, "merge": merge_ms / args.steps,
                                   "fp8_candidates_rerank": fp8_ms / args.steps},
