@@ -263,6 +263,12 @@ int sift_mlp_grad(sift_handle_t h, sift_mlp_t mlp, const void* x, const void* y,
  * rows); epoch_loss (host, opts->epochs floats) may be NULL. */
 int sift_mlp_train(sift_handle_t h, sift_mlp_t mlp, const void* x, uint32_t n, const void* base,
                    uint32_t r_n, int type, const sift_mlp_train_opts* opts, float* epoch_loss);
+/* Persist an LSH index (hyperplane or learned: the hash, the sorted tables and
+ * the gather copy) and load it on any handle without rebuilding (the reference
+ * rebuilds per call, src/lsh_h.cu:903-914).  A loaded learned index owns
+ * copies of its networks.  No counterpart in the reference. */
+int sift_lsh_save(sift_lsh_index_t index, const char* path);
+int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* index);
 /* LSH index whose table t hashes with network mlps[t] (all of the same b);
  * the networks must outlive the index.  Query it with sift_lsh_query. */
 int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_n,

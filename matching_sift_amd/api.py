@@ -199,6 +199,25 @@ class LSHIndex:
         self.ix = ix.value
         return self
 
+    def save(self, path):
+        """Write the index (hash, sorted tables, gather copy) to a file."""
+        check(lib().sift_lsh_save(self.ix, str(path).encode()), self.handle.h)
+
+    @classmethod
+    def load(cls, handle, path):
+        """An index from sift_lsh_save, ready to query (no rebuild); a learned
+        index owns copies of its networks."""
+        self = cls.__new__(cls)
+        self.handle = handle
+        self._keep = None
+        ix = C.c_void_p()
+        check(lib().sift_lsh_load(handle.h, str(path).encode(), C.byref(ix)), handle.h)
+        self.ix = ix.value
+        n, b, L = C.c_uint32(), C.c_int(), C.c_int()
+        check(lib().sift_lsh_info(self.ix, C.byref(n), C.byref(b), C.byref(L)), handle.h)
+        self.n, self.nbits, self.L = n.value, b.value, L.value
+        return self
+
     def query_fallback(self, q, bf_index, min_cand=32, ratio=0.8, type_=None, outputs=None, accept_ratio=0.0):
         """Query, then exact brute force for the queries with fewer than
         min_cand candidates or (accept_ratio > 0) whose best candidate fails

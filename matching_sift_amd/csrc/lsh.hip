@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
 #include "fused.h"
@@ -44,7 +45,9 @@ struct sift_lsh_index {
     uint64_t seed = 0;
     std::vector<float> planes;  // [L][nbits][128]
     PackedSet hp;               // hyperplanes packed as B operands, 32 per table
-    std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller)
+    std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller,
+                                  // or by the index when it was loaded from a file)
+    bool owns_mlps = false;
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
     DevBuf offs;                // b <= OFFS_MAX_BITS: bucket starts, [L][2^b + 1] absolute positions
     DevBuf base_u8, base_nrm;   // gather copy
@@ -328,6 +331,8 @@ int hash_codes(sift_context* h, const sift_lsh_index* ix, const PackedSet& xs, u
 }
 
 void free_index(sift_lsh_index* ix) {
+    if (ix->owns_mlps)
+        for (sift_mlp* m : ix->mlps) sift_mlp_destroy(m);
     for (DevBuf* b : {&ix->keys, &ix->vals, &ix->offs, &ix->base_u8, &ix->base_nrm, &ix->scratch_k,
                       &ix->scratch_v, &ix->hist, &ix->hp.tiles, &ix->hp.norm, &ix->hp.norm_lane})
         release(*b);
@@ -364,6 +369,37 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
         }
     }
     offs[i] = lo;
+}
+
+// The hyperplanes of every table as MFMA B operands: 32 rows per table,
+// planes [nbits, 32) zero.
+int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
+    const int L = ix->L, nbits = ix->nbits;
+    std::vector<float> hp((size_t)L * 32 * 128, 0.f);
+    for (int t = 0; t < L; t++)
+        std::copy(ix->planes.begin() + (size_t)t * nbits * 128, ix->planes.begin() + (size_t)(t + 1) * nbits * 128,
+                  hp.begin() + (size_t)t * 32 * 128);
+    DevBuf hp_dev;
+    SIFT_TRY(reserve(h, hp_dev, hp.size() * 4));
+    int rc = SIFT_OK;
+    if (hipMemcpyAsync(hp_dev.p, hp.data(), hp.size() * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = set_error(h, SIFT_ERR_HIP, "copy hyperplanes");
+    if (rc == SIFT_OK)
+        rc = pack_rows(h, hp_dev.p, SIFT_FLOAT_DEVICE, (uint32_t)L * 32, (uint32_t)L * 32, KIND_BF16, PACK_PLAIN,
+                       ix->hp, flag);
+    if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = set_error(h, SIFT_ERR_HIP, "sync");
+    release(hp_dev);
+    return rc;
+}
+
+int build_offsets(sift_context* h, sift_lsh_index* ix) {
+    if (ix->nbits > OFFS_MAX_BITS || ix->n == 0) return SIFT_OK;
+    const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
+    SIFT_TRY(reserve(h, ix->offs, no * 4));
+    bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
+        static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L, static_cast<uint32_t*>(ix->offs.p));
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
 }
 
 // Codes of every base row for all tables, stable radix sort of
@@ -408,13 +444,7 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
             std::swap(ix->keys, ix->scratch_k);
             std::swap(ix->vals, ix->scratch_v);
         }
-        if (nbits <= OFFS_MAX_BITS && r_n > 0) {
-            const size_t no = (size_t)L * ((1u << nbits) + 1u);
-            if ((rc = reserve(h, ix->offs, no * 4)) != SIFT_OK) break;
-            bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
-                static_cast<const uint32_t*>(ix->keys.p), r_n, nbits, (uint32_t)L, static_cast<uint32_t*>(ix->offs.p));
-            if (hipGetLastError() != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "bucket offsets"); break; }
-        }
+        if ((rc = build_offsets(h, ix)) != SIFT_OK) break;
         if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "sync"); break; }
         release(ix->scratch_k);
         release(ix->scratch_v);
@@ -467,23 +497,7 @@ int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int n
         if ((rc = reserve(h, sc->flag, 16)) != SIFT_OK) break;
         int* flag = static_cast<int*>(sc->flag.p);
         if (hipMemsetAsync(flag, 0, 8, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
-        // hyperplanes: 32 rows per table, planes [nbits, 32) zero
-        std::vector<float> hp((size_t)L * 32 * 128, 0.f);
-        for (int t = 0; t < L; t++)
-            std::copy(ix->planes.begin() + (size_t)t * nbits * 128, ix->planes.begin() + (size_t)(t + 1) * nbits * 128,
-                      hp.begin() + (size_t)t * 32 * 128);
-        DevBuf hp_dev;
-        if ((rc = reserve(h, hp_dev, hp.size() * 4)) != SIFT_OK) break;
-        if (hipMemcpyAsync(hp_dev.p, hp.data(), hp.size() * 4, hipMemcpyHostToDevice, h->stream) != hipSuccess) {
-            release(hp_dev);
-            rc = set_error(h, SIFT_ERR_HIP, "copy hyperplanes");
-            break;
-        }
-        rc = pack_rows(h, hp_dev.p, SIFT_FLOAT_DEVICE, (uint32_t)L * 32, (uint32_t)L * 32, KIND_BF16,
-                       PACK_PLAIN, ix->hp, flag + 1);
-        if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = set_error(h, SIFT_ERR_HIP, "sync");
-        release(hp_dev);
-        if (rc != SIFT_OK) break;
+        if ((rc = pack_planes(h, ix, flag + 1)) != SIFT_OK) break;
         rc = build_tables(h, ix, r, type, flag);
     } while (0);
     if (rc != SIFT_OK) {
@@ -662,6 +676,152 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
                                                             om, idx, dist, matches);
     SIFT_HIP_TRY(h, hipGetLastError());
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return SIFT_OK;
+}
+
+// ------------------------------------------------------- persistence ----
+// File: header, then the hash (hyperplanes [L][nbits][128] float32, or the
+// L networks' parameters), the sorted keys and rows [L*n] uint32, the uint8
+// gather copy [n][128] and its int32 norms [n].  Bucket offsets are rebuilt
+// on load.  Native byte order.
+namespace {
+struct LshFileHeader {
+    char magic[8];
+    uint32_t version, n;
+    int32_t nbits, L, hash_kind;
+    uint32_t nparam;  // learned: parameters per network; else 0
+    uint64_t seed;
+};
+constexpr char LSH_MAGIC[8] = {'S', 'I', 'F', 'T', 'L', 'S', 'H', '1'};
+
+size_t lsh_file_bytes(const LshFileHeader& hd) {
+    const size_t hash = hd.hash_kind == SIFT_LSH_LEARNED ? (size_t)hd.L * hd.nparam * 4
+                                                          : (size_t)hd.L * hd.nbits * 128 * 4;
+    return sizeof(LshFileHeader) + hash + (size_t)hd.L * hd.n * 8 + (size_t)hd.n * 132;
+}
+}  // namespace
+
+int sift_lsh_save(sift_lsh_index_t ix, const char* path) {
+    if (!ix || !path) return set_error(ix ? ix->h : nullptr, SIFT_ERR_INVALID, "null index/path");
+    sift_context* h = ix->h;
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    LshFileHeader hd{};
+    std::memcpy(hd.magic, LSH_MAGIC, 8);
+    hd.version = 1;
+    hd.n = ix->n;
+    hd.nbits = ix->nbits;
+    hd.L = ix->L;
+    hd.hash_kind = ix->hash_kind;
+    hd.nparam = ix->hash_kind == SIFT_LSH_LEARNED ? (uint32_t)SIFT_MLP_NPARAM : 0u;
+    hd.seed = ix->seed;
+    std::vector<float> hash;
+    if (ix->hash_kind == SIFT_LSH_LEARNED) {
+        hash.resize((size_t)ix->L * SIFT_MLP_NPARAM);
+        for (int t = 0; t < ix->L; t++) SIFT_TRY(sift_mlp_get_params(ix->mlps[t], hash.data() + (size_t)t * SIFT_MLP_NPARAM));
+    } else {
+        hash = ix->planes;
+    }
+    const size_t nk = (size_t)ix->L * ix->n;
+    std::vector<uint32_t> keys(nk), vals(nk);
+    std::vector<uint8_t> u8((size_t)ix->n * 128);
+    std::vector<int32_t> nrm(ix->n);
+    if (nk) {
+        SIFT_HIP_TRY(h, hipMemcpy(keys.data(), ix->keys.p, nk * 4, hipMemcpyDeviceToHost));
+        SIFT_HIP_TRY(h, hipMemcpy(vals.data(), ix->vals.p, nk * 4, hipMemcpyDeviceToHost));
+    }
+    if (ix->n) {
+        SIFT_HIP_TRY(h, hipMemcpy(u8.data(), ix->base_u8.p, u8.size(), hipMemcpyDeviceToHost));
+        SIFT_HIP_TRY(h, hipMemcpy(nrm.data(), ix->base_nrm.p, nrm.size() * 4, hipMemcpyDeviceToHost));
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return set_error(h, SIFT_ERR_INVALID, std::string("cannot write ") + path);
+    bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1;
+    ok = ok && (hash.empty() || std::fwrite(hash.data(), 4, hash.size(), f) == hash.size());
+    ok = ok && (nk == 0 || (std::fwrite(keys.data(), 4, nk, f) == nk && std::fwrite(vals.data(), 4, nk, f) == nk));
+    ok = ok && (ix->n == 0 || (std::fwrite(u8.data(), 1, u8.size(), f) == u8.size() &&
+                               std::fwrite(nrm.data(), 4, nrm.size(), f) == nrm.size()));
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? SIFT_OK : set_error(h, SIFT_ERR_INVALID, std::string("write failed: ") + path);
+}
+
+int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* out) {
+    if (!h || !path || !out) return set_error(h, SIFT_ERR_INVALID, "null handle/path/index");
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return set_error(h, SIFT_ERR_INVALID, std::string("cannot open ") + path);
+    struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+    LshFileHeader hd{};
+    if (std::fread(&hd, sizeof hd, 1, f) != 1 || std::memcmp(hd.magic, LSH_MAGIC, 8) != 0 || hd.version != 1)
+        return set_error(h, SIFT_ERR_INVALID, std::string(path) + " is not an LSH index file");
+    int tbits = 0;
+    while (hd.L > 0 && (1 << tbits) < hd.L) tbits++;
+    const bool learned = hd.hash_kind == SIFT_LSH_LEARNED;
+    if (hd.nbits < 1 || hd.nbits > 32 || hd.L < 1 || hd.L > 256 || hd.nbits + tbits > 32 ||
+        (uint64_t)hd.n * (uint64_t)hd.L >= (1ull << 32) ||
+        (hd.hash_kind != SIFT_LSH_HYPERPLANE && hd.hash_kind != SIFT_LSH_HYPERPLANE_PM1 && !learned) ||
+        (learned && hd.nparam != (uint32_t)SIFT_MLP_NPARAM))
+        return set_error(h, SIFT_ERR_INVALID, std::string(path) + ": bad index header");
+    std::fseek(f, 0, SEEK_END);
+    const long size = std::ftell(f);
+    if (size < 0 || (size_t)size != lsh_file_bytes(hd))
+        return set_error(h, SIFT_ERR_INVALID, std::string(path) + ": size does not match its header");
+    std::fseek(f, (long)sizeof hd, SEEK_SET);
+    std::vector<float> hash(learned ? (size_t)hd.L * hd.nparam : (size_t)hd.L * hd.nbits * 128);
+    const size_t nk = (size_t)hd.L * hd.n;
+    std::vector<uint32_t> keys(nk), vals(nk);
+    std::vector<uint8_t> u8((size_t)hd.n * 128);
+    std::vector<int32_t> nrm(hd.n);
+    bool ok = std::fread(hash.data(), 4, hash.size(), f) == hash.size();
+    ok = ok && (nk == 0 || (std::fread(keys.data(), 4, nk, f) == nk && std::fread(vals.data(), 4, nk, f) == nk));
+    ok = ok && (hd.n == 0 || (std::fread(u8.data(), 1, u8.size(), f) == u8.size() &&
+                              std::fread(nrm.data(), 4, nrm.size(), f) == nrm.size()));
+    if (!ok) return set_error(h, SIFT_ERR_INVALID, std::string("short read: ") + path);
+
+    auto* ix = new sift_lsh_index();
+    ix->h = h;
+    ix->n = hd.n;
+    ix->nbits = hd.nbits;
+    ix->L = hd.L;
+    ix->hash_kind = hd.hash_kind;
+    ix->seed = hd.seed;
+    int rc = SIFT_OK;
+    do {
+        if (learned) {
+            ix->owns_mlps = true;
+            for (int t = 0; t < hd.L && rc == SIFT_OK; t++) {
+                sift_mlp* m = nullptr;
+                if ((rc = sift_mlp_create(h, hd.nbits, 0, &m)) != SIFT_OK) break;
+                ix->mlps.push_back(m);
+                rc = sift_mlp_set_params(m, hash.data() + (size_t)t * hd.nparam);
+            }
+            if (rc != SIFT_OK) break;
+        } else {
+            ix->planes = hash;
+            LshScratch* sc = scratch_of(h);
+            if ((rc = reserve(h, sc->flag, 16)) != SIFT_OK) break;
+            if (hipMemsetAsync(sc->flag.p, 0, 8, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
+            if ((rc = pack_planes(h, ix, static_cast<int*>(sc->flag.p) + 1)) != SIFT_OK) break;
+        }
+        for (DevBuf* b : {&ix->keys, &ix->vals})
+            if ((rc = reserve(h, *b, std::max<size_t>(nk, 1) * 4)) != SIFT_OK) break;
+        if (rc != SIFT_OK) break;
+        if ((rc = reserve(h, ix->base_u8, std::max<size_t>(u8.size(), 1))) != SIFT_OK) break;
+        if ((rc = reserve(h, ix->base_nrm, std::max<size_t>(nrm.size(), 1) * 4)) != SIFT_OK) break;
+        if ((nk && (hipMemcpy(ix->keys.p, keys.data(), nk * 4, hipMemcpyHostToDevice) != hipSuccess ||
+                    hipMemcpy(ix->vals.p, vals.data(), nk * 4, hipMemcpyHostToDevice) != hipSuccess)) ||
+            (hd.n && (hipMemcpy(ix->base_u8.p, u8.data(), u8.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                      hipMemcpy(ix->base_nrm.p, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice) != hipSuccess))) {
+            rc = set_error(h, SIFT_ERR_HIP, "copy index to the device");
+            break;
+        }
+        if ((rc = build_offsets(h, ix)) != SIFT_OK) break;
+        if (hipStreamSynchronize(h->stream) != hipSuccess) rc = set_error(h, SIFT_ERR_HIP, "sync");
+    } while (0);
+    if (rc != SIFT_OK) {
+        free_index(ix);
+        return rc;
+    }
+    *out = ix;
     return SIFT_OK;
 }
 

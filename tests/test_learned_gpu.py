@@ -201,3 +201,23 @@ def test_fallback_is_exact(handle):
     ix.close()
     bf.close()
     mlp.close()
+
+
+def test_learned_index_save_load(handle, tmp_path):
+    """A learned index saved and loaded after its networks are gone (the
+    loaded index owns copies) answers exactly as before."""
+    b = datagen.base(6_000, seed=83)
+    q = datagen.queries(b, 400, seed=84)
+    mlps = [HashMLP(handle, 14, seed=300 + t) for t in range(2)]
+    ix = LSHIndex.learned(handle, b, mlps)
+    say("learned index save/load")
+    ref = ix.query(q)
+    ix.save(tmp_path / "learned.lsh")
+    ix.close()
+    for m in mlps:
+        m.close()
+    jx = LSHIndex.load(handle, tmp_path / "learned.lsh")
+    assert (jx.n, jx.nbits, jx.L) == (len(b), 14, 2)
+    for x, y in zip(ref, jx.query(q)):
+        np.testing.assert_array_equal(x, y)
+    jx.close()

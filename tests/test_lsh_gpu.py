@@ -114,3 +114,32 @@ def test_lsh_rejects_bad_params(handle):
     for nbits, L in ((0, 1), (33, 1), (8, 0), (31, 4)):
         with pytest.raises(SiftError):
             LSHIndex(handle, b, nbits, L)
+
+
+def test_lsh_save_load(handle, tmp_path):
+    """A saved index loads on the handle without a rebuild and answers
+    exactly as before (direct bucket offsets at b = 12, binary search at
+    b = 24); damaged files are rejected."""
+    b = datagen.base(12_000, seed=61)
+    q = datagen.queries(b, 300, seed=62)
+    for nbits, L in ((12, 3), (24, 2)):
+        say("lsh save/load", nbits, L)
+        ix = LSHIndex(handle, b, nbits, L, seed=5)
+        ref = ix.query(q)
+        planes = ix.hyperplanes()
+        p = tmp_path / ("ix%d.lsh" % nbits)
+        ix.save(p)
+        ix.close()
+        jx = LSHIndex.load(handle, p)
+        assert (jx.n, jx.nbits, jx.L) == (len(b), nbits, L)
+        np.testing.assert_array_equal(jx.hyperplanes(), planes)
+        for x, y in zip(ref, jx.query(q)):
+            np.testing.assert_array_equal(x, y)
+        jx.close()
+    (tmp_path / "bad.lsh").write_bytes(b"not an index")
+    with pytest.raises(SiftError):
+        LSHIndex.load(handle, tmp_path / "bad.lsh")
+    data = (tmp_path / "ix12.lsh").read_bytes()
+    (tmp_path / "cut.lsh").write_bytes(data[:-7])
+    with pytest.raises(SiftError):
+        LSHIndex.load(handle, tmp_path / "cut.lsh")
