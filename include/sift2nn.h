@@ -263,6 +263,15 @@ int sift_mlp_grad(sift_handle_t h, sift_mlp_t mlp, const void* x, const void* y,
  * rows); epoch_loss (host, opts->epochs floats) may be NULL. */
 int sift_mlp_train(sift_handle_t h, sift_mlp_t mlp, const void* x, uint32_t n, const void* base,
                    uint32_t r_n, int type, const sift_mlp_train_opts* opts, float* epoch_loss);
+/* Multi-probe (no counterpart in the reference): every query also looks up,
+ * per table, the buckets of its code with each of the `probes` least
+ * confident bits flipped (smallest |h.x| or |z| first; 0 = off, at most
+ * min(nbits, 16)).  A setting of the index used by every later query. */
+int sift_lsh_set_probes(sift_lsh_index_t index, int probes);
+/* The codes a query looks up, device [L][probes + 1][q_n]: per table the
+ * base code, then the probes in order. */
+int sift_lsh_probe_codes(sift_handle_t h, sift_lsh_index_t index, const void* q_points, int type,
+                         uint32_t q_n, uint32_t* codes);
 /* Persist an LSH index (hyperplane or learned: the hash, the sorted tables and
  * the gather copy) and load it on any handle without rebuilding (the reference
  * rebuilds per call, src/lsh_h.cu:903-914).  A loaded learned index owns

@@ -72,6 +72,8 @@ SIGNATURES = {
     "sift_mlp_grad": (i32, [vp, vp, vp, vp, i32, u32, vp, vp, vp, vp]),
     "sift_mlp_train": (i32, [vp, vp, vp, u32, vp, u32, i32, vp, vp]),
     "sift_lsh_build_learned": (i32, [vp, vp, i32, u32, vp, i32, C.POINTER(vp)]),
+    "sift_lsh_set_probes": (i32, [vp, i32]),
+    "sift_lsh_probe_codes": (i32, [vp, vp, vp, i32, u32, vp]),
     "sift_lsh_save": (i32, [vp, C.c_char_p]),
     "sift_lsh_load": (i32, [vp, C.c_char_p, C.POINTER(vp)]),
     "sift_lsh_query_fallback": (i32, [vp, vp, vp, vp, i32, u32, u32, f32, vp, vp, vp, vp, f32, vp]),

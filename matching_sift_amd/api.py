@@ -199,6 +199,21 @@ class LSHIndex:
         self.ix = ix.value
         return self
 
+    def set_probes(self, probes):
+        """Multi-probe: also look up the buckets with each of the `probes`
+        least confident bits flipped (0 = off)."""
+        check(lib().sift_lsh_set_probes(self.ix, int(probes)), self.handle.h)
+        self.probes = int(probes)
+
+    def probe_codes(self, q, type_=None):
+        """[L][probes + 1][q] codes looked up per query (base code first)."""
+        qd = to_device(q)
+        P = getattr(self, "probes", 0)
+        out = DeviceArray.empty((self.L, P + 1, qd.shape[0]), np.uint32)
+        check(lib().sift_lsh_probe_codes(self.handle.h, self.ix, qd.ptr, _input_type(qd, type_), qd.shape[0],
+                                         out.ptr), self.handle.h)
+        return out.numpy()
+
     def save(self, path):
         """Write the index (hash, sorted tables, gather copy) to a file."""
         check(lib().sift_lsh_save(self.ix, str(path).encode()), self.handle.h)

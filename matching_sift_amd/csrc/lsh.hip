@@ -48,6 +48,7 @@ struct sift_lsh_index {
     std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller,
                                   // or by the index when it was loaded from a file)
     bool owns_mlps = false;
+    int probes = 0;  // multi-probe: extra buckets per table (the least confident bits flipped)
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
     DevBuf offs;                // b <= OFFS_MAX_BITS: bucket starts, [L][2^b + 1] absolute positions
     DevBuf base_u8, base_nrm;   // gather copy
@@ -59,6 +60,7 @@ namespace {
 // Per-handle query scratch.
 struct LshScratch {
     DevBuf q_u8, q_nrm, qcodes, flag;
+    DevBuf qbase, qmarg;  // multi-probe: base codes [L][q_n] and margins [L][q_n][32]
     PackedSet q_tiles;
     // exact fallback
     DevBuf fb_nc, fb_dist, fb_list, fb_count, fb_map, fb_out;
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void lsh_project(const uint4* __restrict__ x_t
                                                    uint32_t n_rows, uint32_t n_tiles,
                                                    const uint4* __restrict__ h_tiles, int L,
                                                    uint32_t* __restrict__ codes, uint32_t stride,
-                                                   int b, int combine) {
+                                                   int b, int combine, float* __restrict__ margins) {
     const int lane = threadIdx.x & 63;
     const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (tile >= n_tiles) return;  // whole wave
@@ -105,6 +107,13 @@ __global__ __launch_bounds__(256) void lsh_project(const uint4* __restrict__ x_t
         }
         if (lane < 32 && row < n_rows)
             codes[(size_t)t * stride + row] = (combine && b < 32) ? (((uint32_t)t << b) | code) : code;
+        if (margins) {  // |h.x| of hyperplane lane % 32 for the lane's 16 rows: [t][row][32]
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const uint32_t r = tile * 32 + 8 * (j >> 2) + 4 * (lane >> 5) + (j & 3);
+                if (r < n_rows) margins[((size_t)t * stride + r) * 32 + (lane & 31)] = fabsf(acc[j]);
+            }
+        }
     }
 }
 
@@ -154,10 +163,10 @@ struct RerankArgs {
     const uint8_t* q_u8;
     const uint32_t* keys;
     const uint32_t* vals;
-    const uint32_t* qcodes;  // [L][q_n]
+    const uint32_t* qcodes;  // [L][P + 1][q_n]: per table the base code, then the probes
     const uint32_t* offs;    // bucket starts [L][2^b + 1], or null (binary search)
     uint32_t n, q_n;
-    int L, b;
+    int L, b, P;
     uint32_t* idx;
     float* dist;
     uint32_t* ncand;
@@ -212,9 +221,10 @@ __global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
     nq += __shfl_xor(nq, 4);
     uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
     uint32_t nc = 0;
-    for (int t = 0; t < a.L; t++) {
+    for (int tp = 0; tp < a.L * (a.P + 1); tp++) {
+        const int t = tp / (a.P + 1);
         const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
-        const uint32_t code = a.qcodes[(size_t)t * a.q_n + q];
+        const uint32_t code = a.qcodes[(size_t)tp * a.q_n + q];
         uint32_t lo, hi;
         if (a.offs) {
             const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
@@ -311,22 +321,84 @@ void make_planes(std::vector<float>& p, int L, int nbits, int kind, uint64_t see
 }
 
 int project(sift_context* h, const PackedSet& xs, uint32_t n, const PackedSet& hp, int L, int b,
-            uint32_t* codes, uint32_t stride, bool combine) {
+            uint32_t* codes, uint32_t stride, bool combine, float* margins = nullptr) {
     if (n == 0) return SIFT_OK;
     if (xs.ntiles == 0) return SIFT_OK;
     const unsigned grid = (xs.ntiles + 3) / 4;
     lsh_project<<<grid, 256, 0, h->stream>>>(static_cast<const uint4*>(xs.tiles.p), n, xs.ntiles,
                                              static_cast<const uint4*>(hp.tiles.p), L, codes, stride,
-                                             b, combine ? 1 : 0);
+                                             b, combine ? 1 : 0, margins);
     SIFT_HIP_TRY(h, hipGetLastError());
     return SIFT_OK;
 }
 
 // Codes of every table: hyperplanes (one launch) or one network per table.
+// margins (optional, [L][stride][32]): |h.x| or |z| per bit, signed z for
+// the learned kind (the probe kernel takes the magnitude).
 int hash_codes(sift_context* h, const sift_lsh_index* ix, const PackedSet& xs, uint32_t n, uint32_t* codes,
-               uint32_t stride, bool combine) {
-    if (ix->hash_kind != SIFT_LSH_LEARNED) return project(h, xs, n, ix->hp, ix->L, ix->nbits, codes, stride, combine);
-    for (int t = 0; t < ix->L; t++) SIFT_TRY(mlp_forward(ix->mlps[t], xs, n, nullptr, nullptr, codes, stride, t, combine));
+               uint32_t stride, bool combine, float* margins = nullptr) {
+    if (ix->hash_kind != SIFT_LSH_LEARNED)
+        return project(h, xs, n, ix->hp, ix->L, ix->nbits, codes, stride, combine, margins);
+    for (int t = 0; t < ix->L; t++)
+        SIFT_TRY(mlp_forward(ix->mlps[t], xs, n, nullptr, margins ? margins + (size_t)t * stride * 32 : nullptr, codes,
+                             stride, t, combine));
+    return SIFT_OK;
+}
+
+// Multi-probe codes: per (table, query) the base code, then the code with
+// each of the P least confident bits flipped (smallest |margin| first, lower
+// bit first on ties): pcodes[(t * (P + 1) + p) * q_n + q].
+__global__ void make_probes(const uint32_t* __restrict__ codes, const float* __restrict__ margins, uint32_t q_n,
+                            int L, int b, int P, uint32_t* __restrict__ pcodes) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)L * q_n) return;
+    const uint32_t t = (uint32_t)(i / q_n), q = (uint32_t)(i % q_n);
+    const uint32_t code = codes[i];
+    const float* m = margins + i * 32;
+    float mv[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) mv[k] = k < b ? fabsf(m[k]) : __builtin_huge_valf();
+    uint32_t* out = pcodes + (size_t)t * (P + 1) * q_n + q;
+    out[0] = code;
+    uint32_t used = 0;
+    for (int p = 1; p <= P; p++) {
+        int best = -1;
+        float bv = __builtin_huge_valf();
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            const bool free_bit = !((used >> k) & 1u) && k < b;
+            if (free_bit && (best < 0 || mv[k] < bv)) {
+                best = k;
+                bv = mv[k];
+            }
+        }
+        used |= 1u << best;
+        out[(size_t)p * q_n] = code ^ (1u << best);
+    }
+}
+
+constexpr int MAX_PROBES = 16;
+
+// Query codes for the re-rank, [L][P + 1][q_n], from the packed queries in
+// sc->q_tiles: the base codes alone without probes, else base codes and
+// margins, then make_probes.
+int query_codes(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, uint32_t q_n, const uint32_t** out) {
+    const int L = ix->L, P = ix->probes;
+    SIFT_TRY(reserve(h, sc->qcodes, (size_t)q_n * L * (P + 1) * 4));
+    auto* qcodes = static_cast<uint32_t*>(sc->qcodes.p);
+    if (P == 0) {
+        SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, qcodes, q_n, false));
+    } else {
+        SIFT_TRY(reserve(h, sc->qbase, (size_t)q_n * L * 4));
+        SIFT_TRY(reserve(h, sc->qmarg, (size_t)q_n * L * 32 * 4));
+        auto* base = static_cast<uint32_t*>(sc->qbase.p);
+        SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, base, q_n, false, static_cast<float*>(sc->qmarg.p)));
+        const size_t nt = (size_t)L * q_n;
+        make_probes<<<(unsigned)((nt + 255) / 256), 256, 0, h->stream>>>(base, static_cast<const float*>(sc->qmarg.p),
+                                                                         q_n, L, ix->nbits, P, qcodes);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
+    *out = qcodes;
     return SIFT_OK;
 }
 
@@ -459,7 +531,7 @@ extern "C" {
 void sift_lsh_state_free(sift_context* h) {
     if (!h || !h->lsh_state) return;
     auto* s = static_cast<LshScratch*>(h->lsh_state);
-    for (DevBuf* b : {&s->q_u8, &s->q_nrm, &s->qcodes, &s->flag, &s->q_tiles.tiles, &s->q_tiles.norm,
+    for (DevBuf* b : {&s->q_u8, &s->q_nrm, &s->qcodes, &s->qbase, &s->qmarg, &s->flag, &s->q_tiles.tiles, &s->q_tiles.norm,
                       &s->q_tiles.norm_lane, &s->fb_nc, &s->fb_dist, &s->fb_list, &s->fb_count, &s->fb_map, &s->fb_out,
                       &s->fb_q.tiles, &s->fb_q.norm, &s->fb_q.norm_lane})
         release(*b);
@@ -538,11 +610,10 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     SIFT_TRY(to_u8(h, q, type, q_n, sc->q_u8, sc->q_nrm, flag));
     const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
     SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, flag));
-    SIFT_TRY(reserve(h, sc->qcodes, (size_t)q_n * ix->L * 4));
-    auto* qcodes = static_cast<uint32_t*>(sc->qcodes.p);
+    const uint32_t* qcodes = nullptr;
     {
         ProfScope ps(h, PROF_LSH_PROJ);
-        SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, qcodes, q_n, false));
+        SIFT_TRY(query_codes(h, ix, sc, q_n, &qcodes));
     }
     RerankArgs a;
     a.base_u8 = static_cast<const uint8_t*>(ix->base_u8.p);
@@ -556,6 +627,7 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     a.q_n = q_n;
     a.L = ix->L;
     a.b = ix->nbits;
+    a.P = ix->probes;
     a.idx = idx;
     a.dist = dist;
     a.ncand = ncand;
@@ -567,6 +639,33 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     if (blocking) SIFT_TRY(check_flag(h, flag, "sift_lsh_query"));
+    return SIFT_OK;
+}
+
+int sift_lsh_set_probes(sift_lsh_index_t ix, int probes) {
+    if (!ix) return set_error(nullptr, SIFT_ERR_INVALID, "null index");
+    if (probes < 0 || probes > std::min(ix->nbits, MAX_PROBES))
+        return set_error(ix->h, SIFT_ERR_INVALID, "probes must be in [0, min(nbits, 16)]");
+    ix->probes = probes;
+    return SIFT_OK;
+}
+
+int sift_lsh_probe_codes(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type, uint32_t q_n,
+                         uint32_t* codes) {
+    if (!h || !ix) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
+    if (type < SIFT_FLOAT_HOST || type > SIFT_U8_DEVICE) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
+    if (q_n == 0) return SIFT_OK;
+    if (!q || !codes) return set_error(h, SIFT_ERR_INVALID, "null pointer");
+    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
+    LshScratch* sc = scratch_of(h);
+    SIFT_TRY(reserve(h, sc->flag, 16));
+    const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
+    SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, static_cast<int*>(sc->flag.p)));
+    const uint32_t* pc = nullptr;
+    SIFT_TRY(query_codes(h, ix, sc, q_n, &pc));
+    SIFT_HIP_TRY(h, hipMemcpyAsync(codes, pc, (size_t)ix->L * (ix->probes + 1) * q_n * 4, hipMemcpyDeviceToDevice,
+                                   h->stream));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return SIFT_OK;
 }
 
@@ -680,6 +779,7 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
 }
 
 // ------------------------------------------------------- persistence ----
+// (the probe count is a query setting, not saved)
 // File: header, then the hash (hyperplanes [L][nbits][128] float32, or the
 // L networks' parameters), the sorted keys and rows [L*n] uint32, the uint8
 // gather copy [n][128] and its int32 norms [n].  Bucket offsets are rebuilt

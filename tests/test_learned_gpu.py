@@ -221,3 +221,37 @@ def test_learned_index_save_load(handle, tmp_path):
     for x, y in zip(ref, jx.query(q)):
         np.testing.assert_array_equal(x, y)
     jx.close()
+
+
+def test_learned_multiprobe(handle):
+    """Learned multi-probe: the probes flip the bits of smallest |z| (the
+    network's own outputs, lower bit on ties) and the index answers as the
+    oracle's LSH query over the probe codes; more probes find a query's
+    nearest row at least as often."""
+    nbits, L, P = 14, 2, 4
+    b = datagen.base(6_000, seed=85)
+    q = datagen.queries(b, 500, seed=86)
+    mlps = [HashMLP(handle, nbits, seed=400 + t) for t in range(L)]
+    ix = LSHIndex.learned(handle, b, mlps)
+    base_idx = ix.query(q)[0]
+    ix.set_probes(P)
+    say("learned multiprobe")
+    pc = ix.probe_codes(q)
+    for t in range(L):
+        codes, z = mlps[t].codes(q, with_z=True)
+        np.testing.assert_array_equal(pc[t, 0], codes)
+        flips = pc[t, 1:] ^ codes[None]
+        assert ((flips != 0) & ((flips & (flips - 1)) == 0)).all()
+        want = np.argsort(np.abs(z[:, :nbits]), axis=1, kind="stable")[:, :P]
+        np.testing.assert_array_equal(np.log2(flips).astype(np.int64).T, want)
+    cb = np.stack([m.codes(b) for m in mlps])
+    cq = pc.reshape(L * (P + 1), -1)
+    cr = np.repeat(cb, P + 1, axis=0)
+    oi, od, _ = oracle.lsh_query(q, b, cq, cr)
+    idx, dist, nc, mat = ix.query(q)
+    np.testing.assert_array_equal(idx, oi)
+    nn = knn2(handle, q, b, compute=COMPUTE_BF16)[0][:, 0]
+    assert (idx[:, 0] == nn).mean() >= (base_idx[:, 0] == nn).mean()
+    ix.close()
+    for m in mlps:
+        m.close()

@@ -143,3 +143,42 @@ def test_lsh_save_load(handle, tmp_path):
     (tmp_path / "cut.lsh").write_bytes(data[:-7])
     with pytest.raises(SiftError):
         LSHIndex.load(handle, tmp_path / "cut.lsh")
+
+
+def smallest_bits(margins, P):
+    """Bits of the P smallest margins, lowest bit first on ties ([..., bits])."""
+    return np.argsort(margins, axis=-1, kind="stable")[..., :P]
+
+
+@pytest.mark.parametrize("nbits,L,P", [(12, 2, 3), (16, 1, 5), (24, 2, 2)])
+def test_lsh_multiprobe_matches_oracle(handle, nbits, L, P):
+    """Multi-probe: per table the base code, then the codes with each of the P
+    bits of smallest |h.x| flipped (exact integers here, so ties go to the
+    lower bit); buckets, candidate counts and re-ranked results equal the
+    oracle's LSH query over those codes."""
+    b = datagen.base(12_000, seed=33)
+    q = datagen.queries(b, 300, seed=34)
+    ix = LSHIndex(handle, b, nbits, L, seed=9)
+    ix.set_probes(P)
+    say("multiprobe", nbits, L, P)
+    pc = ix.probe_codes(q)
+    base = ix.codes(q)
+    np.testing.assert_array_equal(pc[:, 0], base)
+    H = ix.hyperplanes()
+    for t in range(L):
+        marg = np.abs(q.astype(np.float64) @ H[t].astype(np.float64).T)  # [Q][nbits], exact integers
+        flips = pc[t, 1:] ^ base[t][None]  # [P][Q]: one bit each
+        assert ((flips != 0) & ((flips & (flips - 1)) == 0)).all()
+        np.testing.assert_array_equal(np.log2(flips).astype(np.int64).T, smallest_bits(marg, P))
+    cb = ix.codes(b)
+    cq = pc.reshape(L * (P + 1), -1)
+    cr = np.repeat(cb, P + 1, axis=0)
+    oi, od, _ = oracle.lsh_query(q, b, cq, cr)
+    idx, dist, nc, mat = ix.query(q)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_array_equal(dist, np.where(oi == 0xFFFFFFFF, FMAX, od.astype(np.float32)))
+    want_nc = sum((cr[k][None, :] == cq[k][:, None]).sum(1) for k in range(cq.shape[0]))
+    np.testing.assert_array_equal(nc, want_nc.astype(np.uint32))
+    with pytest.raises(SiftError):
+        ix.set_probes(min(nbits, 16) + 1)
+    ix.close()
