@@ -240,31 +240,38 @@ def learned_bench(args, ms, h):
     tables = [int(x) for x in args.learned_tables.split(",")]
     rows = []
 
+    probes = [int(x) for x in args.learned_probes.split(",")]
+
     def measure(kind, nb, L, ix, build_s, train_s):
-        for rule, acc in (("count", 0.0), ("ratio", 0.8)):
-            for _ in range(args.warmup):
-                ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE, accept_ratio=acc)
-            h.sync()
-            h.profile(True)
-            h.profile_reset()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                idx, _, nc, mat, nf = ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE,
-                                                        accept_ratio=acc)
-            dt = (time.perf_counter() - t0) / args.steps
-            # device time per call: hashing, bucket lookup + re-rank, exact fallback kernels
-            brk = {k: h.profile_read(p)[0] / args.steps for k, p in
-                   (("hash_ms", "lsh_proj"), ("rerank_ms", "lsh_query"), ("fallback_fused_ms", "fused"))}
-            h.profile(False)
-            r1 = float((idx[:, 0] == exact[:, 0]).mean())
-            r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
-            rm = float((mat == exact_m).mean())
-            row = {"workload": "learned", "rule": rule, "kind": kind, "nbits": nb, "tables": L, "value": M / dt,
-                   "unit": "queries/s", "recall_nn1": r1, "recall_2nn": r2, "recall_match": rm,
-                   "mean_candidates": float(nc.mean()), "fallback_queries": int(nf), "build_s": build_s,
-                   "train_s": train_s, "ms_per_call": dt * 1e3, "breakdown": brk}
-            rows.append(row)
-            print(json.dumps(row), flush=True)
+        for P in probes:
+            if P > nb:
+                continue
+            ix.set_probes(P)
+            for rule, acc in (("count", 0.0), ("ratio", 0.8)):
+                for _ in range(args.warmup):
+                    ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE, accept_ratio=acc)
+                h.sync()
+                h.profile(True)
+                h.profile_reset()
+                t0 = time.perf_counter()
+                for _ in range(args.steps):
+                    idx, _, nc, mat, nf = ix.query_fallback(qd, bf, min_cand=32, outputs=outs, type_=ms.FLOAT_DEVICE,
+                                                            accept_ratio=acc)
+                dt = (time.perf_counter() - t0) / args.steps
+                # device time per call: hashing, bucket lookup + re-rank, exact fallback kernels
+                brk = {k: h.profile_read(p)[0] / args.steps for k, p in
+                       (("hash_ms", "lsh_proj"), ("rerank_ms", "lsh_query"), ("fallback_fused_ms", "fused"))}
+                h.profile(False)
+                r1 = float((idx[:, 0] == exact[:, 0]).mean())
+                r2 = float(((idx[:, 0] == exact[:, 0]) & (idx[:, 1] == exact[:, 1])).mean())
+                rm = float((mat == exact_m).mean())
+                row = {"workload": "learned", "rule": rule, "kind": kind, "nbits": nb, "tables": L, "probes": P,
+                       "value": M / dt,
+                       "unit": "queries/s", "recall_nn1": r1, "recall_2nn": r2, "recall_match": rm,
+                       "mean_candidates": float(nc.mean()), "fallback_queries": int(nf), "build_s": build_s,
+                       "train_s": train_s, "ms_per_call": dt * 1e3, "breakdown": brk}
+                rows.append(row)
+                print(json.dumps(row), flush=True)
 
     for nb in bits:
         nets, train_s = [], []
@@ -300,7 +307,8 @@ def learned_bench(args, ms, h):
                     best = max(ok, key=lambda r: r["value"]) if ok else None
                     key = "%s_%s_%s_%.2f" % (rule, kind, rk, target)
                     summary[key] = ({"qps": best["value"], "nbits": best["nbits"], "tables": best["tables"],
-                                     rk: best[rk], "fallback_queries": best["fallback_queries"]} if best else None)
+                                     "probes": best["probes"], rk: best[rk],
+                                     "fallback_queries": best["fallback_queries"]} if best else None)
     print(json.dumps(summary), flush=True)
     bf.close()
 
@@ -369,6 +377,7 @@ def main():
     ap.add_argument("--data", default=os.path.join(ROOT, "data", "sift"), help="ANN_SIFT1M files (--workload sift1m)")
     ap.add_argument("--learned-bits", default="16,20,24")
     ap.add_argument("--learned-tables", default="1,2,4,8,16")
+    ap.add_argument("--learned-probes", default="0,4", help="multi-probe counts per table")
     ap.add_argument("--train-n", type=int, default=200_000)
     ap.add_argument("--train-epochs", type=int, default=10)
     ap.add_argument("--pairs", type=int, default=512)
