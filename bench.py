@@ -386,9 +386,9 @@ def main():
     ap.add_argument("--lsh-tables", default="1,2,4,8")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
-              "2-NN path has no multi-GPU code (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the 2-NN path here runs on a single MI355X and has no "
+              "multi-GPU code (a base split into per-GPU slabs with an all-gather of the 16-byte "
+              "top-2 partials is the intended sharding).", file=sys.stderr)
         sys.exit(2)
 
     import matching_sift_amd as ms
