@@ -422,17 +422,23 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         // masks them.
         uint32_t fmask[TPS];
         bool any_flag = false;
+        // Float kinds keep every key of the stage for the slow path (their
+        // accumulators are live together anyway, and the fragments die after
+        // the MFMAs); I8 recomputes instead (4 MFMAs per tile and query
+        // block), since holding its keys would cost a wave per SIMD.
+        constexpr bool KEEP = KIND != KIND_I8;
+        key_t tv[KEEP ? TPS : 1][QW][16];
 #pragma unroll
         for (int tt = 0; tt < TPS; tt++) {
-            key_t tv[QW][16];
-            tile_keys(tt, tv);
+            auto& tk = tv[KEEP ? tt : 0];
+            tile_keys(tt, tk);
             const bool last = (uint32_t)(ts + tt + 1) * 32u > task.r_lim;
             fmask[tt] = 0;
 #pragma unroll
             for (int qb = 0; qb < QW; qb++) {
                 // minimum of the 16 keys in 8 min/min3 ops, one compare, and
                 // the wave-wide test on the scalar side
-                const key_t* v = tv[qb];
+                const key_t* v = tk[qb];
                 const key_t a0 = min3k(v[0], v[1], v[2]), a1 = min3k(v[3], v[4], v[5]);
                 const key_t a2 = min3k(v[6], v[7], v[8]), a3 = min3k(v[9], v[10], v[11]);
                 const key_t a4 = min3k(v[12], v[13], v[14]);
@@ -443,17 +449,20 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             if (tt >= ntile) fmask[tt] = 0;
             any_flag |= fmask[tt] != 0;
         }
-        // Slow path (rare once the bound is tight): recompute the flagged
-        // query blocks of the flagged tiles from the resident fragments and
-        // insert, 4 rows at a time.
+        // Slow path (rare once the bound is tight): insert the flagged query
+        // blocks of the flagged tiles (kept keys, or recomputed from the
+        // resident fragments), 4 rows at a time.
         if (any_flag) {
 #pragma unroll
             for (int tt = 0; tt < TPS; tt++) {
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) {
                     if (!((fmask[tt] >> qb) & 1u)) continue;
-                    key_t tv[16];
-                    {
+                    key_t t[16];
+                    if constexpr (KEEP) {
+#pragma unroll
+                        for (int j = 0; j < 16; j++) t[j] = tv[tt][qb][j];
+                    } else {
                         acc_t acc;
                         acc_t init{};
                         if constexpr (FLOAT_KEY) {
@@ -466,9 +475,9 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                         for (int j = 0; j < 16; j++) {
                             if constexpr (FLOAT_KEY)
-                                tv[j] = acc[j];
+                                t[j] = acc[j];
                             else
-                                tv[j] = __mul24((int)acc[j], neg2) + nlr[tt][j];
+                                t[j] = __mul24((int)acc[j], neg2) + nlr[tt][j];
                         }
                     }
                     const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
@@ -477,17 +486,17 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                         for (int j = 0; j < 16; j++) {
                             const bool pad = rowbase + 8u * (j >> 2) + (j & 3) >= task.r_lim;
-                            tv[j] = pad ? Key<KIND>::none() : tv[j];
+                            t[j] = pad ? Key<KIND>::none() : t[j];
                         }
                     }
                     const uint32_t old2 = i2[qb];
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
-                        const key_t gm = min(min(tv[4 * c], tv[4 * c + 1]), min(tv[4 * c + 2], tv[4 * c + 3]));
+                        const key_t gm = min(min(t[4 * c], t[4 * c + 1]), min(t[4 * c + 2], t[4 * c + 3]));
                         if (gm < thr[qb]) {
 #pragma unroll
                             for (int e = 0; e < 4; e++) {
-                                const key_t v = tv[4 * c + e];
+                                const key_t v = t[4 * c + e];
                                 const uint32_t row = rowbase + 8u * c + e;
                                 const bool c1 = v < m1[qb], c2 = v < m2[qb];
                                 m2[qb] = c1 ? m1[qb] : (c2 ? v : m2[qb]);
