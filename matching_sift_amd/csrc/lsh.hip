@@ -221,39 +221,73 @@ __global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
     nq += __shfl_xor(nq, 4);
     uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
     uint32_t nc = 0;
-    for (int tp = 0; tp < a.L * (a.P + 1); tp++) {
-        const int t = tp / (a.P + 1);
-        const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
-        const uint32_t code = a.qcodes[(size_t)tp * a.q_n + q];
-        uint32_t lo, hi;
-        if (a.offs) {
-            const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
-            lo = o[0];
-            hi = o[1];
-        } else {
-            const uint32_t key = tkey | code;
-            const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
-            lo = lower_bound_u32(a.keys, t0, t1, key);
-            hi = upper_bound_u32(a.keys, lo, t1, key);
+    // Candidates are walked bucket by bucket, UNR per 8-lane slot at a time
+    // (8 UNR rows in flight per wave), with the next chunk's row ids loaded
+    // before the current rows are used.  Bucket ranges are looked up for up
+    // to 64 (table, probe) pairs at once, one per lane.
+    constexpr int UNR = 8;
+    const int TP = a.L * (a.P + 1);
+    for (int tb = 0; tb < TP; tb += 64) {
+        uint32_t rlo = 0, rhi = 0;
+        if (tb + lane < TP) {
+            const int tp = tb + lane;
+            const int t = tp / (a.P + 1);
+            const uint32_t code = a.qcodes[(size_t)tp * a.q_n + q];
+            if (a.offs) {
+                const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
+                rlo = o[0];
+                rhi = o[1];
+            } else {
+                const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
+                const uint32_t key = tkey | code;
+                const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
+                rlo = lower_bound_u32(a.keys, t0, t1, key);
+                rhi = upper_bound_u32(a.keys, rlo, t1, key);
+            }
         }
-        nc += hi - lo;
-        for (uint32_t p = lo + g; p < hi; p += 8) {
-            const uint32_t j = a.vals[p];
-            const uint4 bv = reinterpret_cast<const uint4*>(a.base_u8 + (size_t)j * 128)[s];
-            uint32_t dot = __builtin_amdgcn_udot4(qv.x, bv.x, 0u, false);
-            dot = __builtin_amdgcn_udot4(qv.y, bv.y, dot, false);
-            dot = __builtin_amdgcn_udot4(qv.z, bv.z, dot, false);
-            dot = __builtin_amdgcn_udot4(qv.w, bv.w, dot, false);
-            dot += __shfl_xor(dot, 1);
-            dot += __shfl_xor(dot, 2);
-            dot += __shfl_xor(dot, 4);
-            const uint32_t d = nq + (uint32_t)a.base_nrm[j] - 2u * dot;
-            if (j != i1 && j != i2) {
-                const bool c1 = lex_less(d, j, m1, i1), c2 = lex_less(d, j, m2, i2);
-                m2 = c1 ? m1 : (c2 ? d : m2);
-                i2 = c1 ? i1 : (c2 ? j : i2);
-                m1 = c1 ? d : m1;
-                i1 = c1 ? j : i1;
+        const int nt = min(64, TP - tb);
+        for (int i = 0; i < nt; i++) {
+            const uint32_t lo = __shfl(rlo, i), hi = __shfl(rhi, i);
+            nc += hi - lo;
+            uint32_t p0 = lo + g;
+            uint32_t jv[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; u++) jv[u] = p0 + 8u * u < hi ? a.vals[p0 + 8u * u] : 0xFFFFFFFFu;
+            while (p0 < hi) {  // jv[0] is a real row here
+                uint4 bv[UNR];
+                int bn[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    const uint32_t j = jv[u] != 0xFFFFFFFFu ? jv[u] : jv[0];
+                    bv[u] = reinterpret_cast<const uint4*>(a.base_u8 + (size_t)j * 128)[s];
+                    bn[u] = a.base_nrm[j];
+                }
+                const uint32_t pn = p0 + 8u * UNR;
+                uint32_t jn[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; u++) jn[u] = pn + 8u * u < hi ? a.vals[pn + 8u * u] : 0xFFFFFFFFu;
+#pragma unroll
+                for (int u = 0; u < UNR; u++) {
+                    uint32_t dot = __builtin_amdgcn_udot4(qv.x, bv[u].x, 0u, false);
+                    dot = __builtin_amdgcn_udot4(qv.y, bv[u].y, dot, false);
+                    dot = __builtin_amdgcn_udot4(qv.z, bv[u].z, dot, false);
+                    dot = __builtin_amdgcn_udot4(qv.w, bv[u].w, dot, false);
+                    dot += __shfl_xor(dot, 1);
+                    dot += __shfl_xor(dot, 2);
+                    dot += __shfl_xor(dot, 4);
+                    const uint32_t d = nq + (uint32_t)bn[u] - 2u * dot;
+                    const uint32_t j = jv[u];
+                    if (j != 0xFFFFFFFFu && j != i1 && j != i2) {
+                        const bool c1 = lex_less(d, j, m1, i1), c2 = lex_less(d, j, m2, i2);
+                        m2 = c1 ? m1 : (c2 ? d : m2);
+                        i2 = c1 ? i1 : (c2 ? j : i2);
+                        m1 = c1 ? d : m1;
+                        i1 = c1 ? j : i1;
+                    }
+                }
+                p0 = pn;
+#pragma unroll
+                for (int u = 0; u < UNR; u++) jv[u] = jn[u];
             }
         }
     }

@@ -150,7 +150,9 @@ def smallest_bits(margins, P):
     return np.argsort(margins, axis=-1, kind="stable")[..., :P]
 
 
-@pytest.mark.parametrize("nbits,L,P", [(12, 2, 3), (16, 1, 5), (24, 2, 2)])
+# (12, 5, 12) and (24, 5, 13): more than 64 (table, probe) lookups per query,
+# through the offset table and through the binary search
+@pytest.mark.parametrize("nbits,L,P", [(12, 2, 3), (16, 1, 5), (24, 2, 2), (12, 5, 12), (24, 5, 13)])
 def test_lsh_multiprobe_matches_oracle(handle, nbits, L, P):
     """Multi-probe: per table the base code, then the codes with each of the P
     bits of smallest |h.x| flipped (exact integers here, so ties go to the
