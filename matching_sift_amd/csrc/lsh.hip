@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -49,6 +50,7 @@ struct sift_lsh_index {
                                   // or by the index when it was loaded from a file)
     bool owns_mlps = false;
     int probes = 0;  // multi-probe: extra buckets per table (the least confident bits flipped)
+    double cand_per_lookup = 0.0;  // mean bucket size seen by a base-like point (re-rank order rule)
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
     DevBuf offs;                // b <= OFFS_MAX_BITS: bucket starts, [L][2^b + 1] absolute positions
     DevBuf base_u8, base_nrm;   // gather copy
@@ -62,6 +64,8 @@ struct LshScratch {
     DevBuf q_u8, q_nrm, qcodes, flag;
     DevBuf qbase, qmarg;  // multi-probe: base codes [L][q_n] and margins [L][q_n][32]
     PackedSet q_tiles;
+    // bucket-ordered re-rank: unit keys / ids (+ sort scratch), partial top-2 and counts
+    DevBuf ukeys, uvals, utk, utv, uhist, ulevels[4], upart, ucnt;
     // exact fallback
     DevBuf fb_nc, fb_dist, fb_list, fb_count, fb_map, fb_out;
     PackedSet fb_q;
@@ -206,12 +210,77 @@ __device__ inline void merge2_dedup(uint32_t& a1, uint32_t& ia1, uint32_t& a2, u
     }
 }
 
-__global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= a.q_n) return;  // whole wave
-    const int g = lane >> 3, s = lane & 7;  // candidate slot, 16-byte segment
-    const uint4 qv = reinterpret_cast<const uint4*>(a.q_u8 + (size_t)q * 128)[s];
+// Bucket range [lo, hi) of (table, probe) tp for query q: the offset table,
+// or a binary search of the sorted keys.
+__device__ inline void bucket_range(const RerankArgs& a, int tp, uint32_t q, uint32_t& lo, uint32_t& hi) {
+    const int t = tp / (a.P + 1);
+    const uint32_t code = a.qcodes[(size_t)tp * a.q_n + q];
+    if (a.offs) {
+        const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
+        lo = o[0];
+        hi = o[1];
+    } else {
+        const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
+        const uint32_t key = tkey | code;
+        const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
+        lo = lower_bound_u32(a.keys, t0, t1, key);
+        hi = upper_bound_u32(a.keys, lo, t1, key);
+    }
+}
+
+// One wave scans the candidates at sorted positions [lo, hi): lane (g, s)
+// takes rows lo + g, lo + g + 8, ... and their 16-byte segment s, UNR rows per
+// slot at a time (8 UNR rows in flight per wave), the next chunk's row ids
+// loaded before the current rows are used.  Each slot keeps its own top-2
+// (lexicographic on (distance, row), rows already held are skipped).
+__device__ inline void scan_bucket(const RerankArgs& a, const uint4& qv, uint32_t nq, uint32_t lo, uint32_t hi,
+                                   int g, int s, uint32_t& m1, uint32_t& i1, uint32_t& m2, uint32_t& i2) {
+    constexpr int UNR = 8;
+    uint32_t p0 = lo + g;
+    uint32_t jv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; u++) jv[u] = p0 + 8u * u < hi ? a.vals[p0 + 8u * u] : 0xFFFFFFFFu;
+    while (p0 < hi) {  // jv[0] is a real row here
+        uint4 bv[UNR];
+        int bn[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            const uint32_t j = jv[u] != 0xFFFFFFFFu ? jv[u] : jv[0];
+            bv[u] = reinterpret_cast<const uint4*>(a.base_u8 + (size_t)j * 128)[s];
+            bn[u] = a.base_nrm[j];
+        }
+        const uint32_t pn = p0 + 8u * UNR;
+        uint32_t jn[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; u++) jn[u] = pn + 8u * u < hi ? a.vals[pn + 8u * u] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) {
+            uint32_t dot = __builtin_amdgcn_udot4(qv.x, bv[u].x, 0u, false);
+            dot = __builtin_amdgcn_udot4(qv.y, bv[u].y, dot, false);
+            dot = __builtin_amdgcn_udot4(qv.z, bv[u].z, dot, false);
+            dot = __builtin_amdgcn_udot4(qv.w, bv[u].w, dot, false);
+            dot += __shfl_xor(dot, 1);
+            dot += __shfl_xor(dot, 2);
+            dot += __shfl_xor(dot, 4);
+            const uint32_t d = nq + (uint32_t)bn[u] - 2u * dot;
+            const uint32_t j = jv[u];
+            if (j != 0xFFFFFFFFu && j != i1 && j != i2) {
+                const bool c1 = lex_less(d, j, m1, i1), c2 = lex_less(d, j, m2, i2);
+                m2 = c1 ? m1 : (c2 ? d : m2);
+                i2 = c1 ? i1 : (c2 ? j : i2);
+                m1 = c1 ? d : m1;
+                i1 = c1 ? j : i1;
+            }
+        }
+        p0 = pn;
+#pragma unroll
+        for (int u = 0; u < UNR; u++) jv[u] = jn[u];
+    }
+}
+
+// Query row segment s and the query's squared norm (all 64 lanes).
+__device__ inline uint32_t load_query(const RerankArgs& a, uint32_t q, int s, uint4& qv) {
+    qv = reinterpret_cast<const uint4*>(a.q_u8 + (size_t)q * 128)[s];
     uint32_t nq = __builtin_amdgcn_udot4(qv.x, qv.x, 0u, false);
     nq = __builtin_amdgcn_udot4(qv.y, qv.y, nq, false);
     nq = __builtin_amdgcn_udot4(qv.z, qv.z, nq, false);
@@ -219,99 +288,109 @@ __global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
     nq += __shfl_xor(nq, 1);
     nq += __shfl_xor(nq, 2);
     nq += __shfl_xor(nq, 4);
-    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
-    uint32_t nc = 0;
-    // Candidates are walked bucket by bucket, UNR per 8-lane slot at a time
-    // (8 UNR rows in flight per wave), with the next chunk's row ids loaded
-    // before the current rows are used.  Bucket ranges are looked up for up
-    // to 64 (table, probe) pairs at once, one per lane.
-    constexpr int UNR = 8;
-    const int TP = a.L * (a.P + 1);
-    for (int tb = 0; tb < TP; tb += 64) {
-        uint32_t rlo = 0, rhi = 0;
-        if (tb + lane < TP) {
-            const int tp = tb + lane;
-            const int t = tp / (a.P + 1);
-            const uint32_t code = a.qcodes[(size_t)tp * a.q_n + q];
-            if (a.offs) {
-                const uint32_t* o = a.offs + (size_t)t * ((1u << a.b) + 1u) + code;
-                rlo = o[0];
-                rhi = o[1];
-            } else {
-                const uint32_t tkey = a.b < 32 ? ((uint32_t)t << a.b) : 0u;  // b == 32 implies L == 1
-                const uint32_t key = tkey | code;
-                const uint32_t t0 = (uint32_t)t * a.n, t1 = t0 + a.n;
-                rlo = lower_bound_u32(a.keys, t0, t1, key);
-                rhi = upper_bound_u32(a.keys, rlo, t1, key);
-            }
-        }
-        const int nt = min(64, TP - tb);
-        for (int i = 0; i < nt; i++) {
-            const uint32_t lo = __shfl(rlo, i), hi = __shfl(rhi, i);
-            nc += hi - lo;
-            uint32_t p0 = lo + g;
-            uint32_t jv[UNR];
-#pragma unroll
-            for (int u = 0; u < UNR; u++) jv[u] = p0 + 8u * u < hi ? a.vals[p0 + 8u * u] : 0xFFFFFFFFu;
-            while (p0 < hi) {  // jv[0] is a real row here
-                uint4 bv[UNR];
-                int bn[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    const uint32_t j = jv[u] != 0xFFFFFFFFu ? jv[u] : jv[0];
-                    bv[u] = reinterpret_cast<const uint4*>(a.base_u8 + (size_t)j * 128)[s];
-                    bn[u] = a.base_nrm[j];
-                }
-                const uint32_t pn = p0 + 8u * UNR;
-                uint32_t jn[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; u++) jn[u] = pn + 8u * u < hi ? a.vals[pn + 8u * u] : 0xFFFFFFFFu;
-#pragma unroll
-                for (int u = 0; u < UNR; u++) {
-                    uint32_t dot = __builtin_amdgcn_udot4(qv.x, bv[u].x, 0u, false);
-                    dot = __builtin_amdgcn_udot4(qv.y, bv[u].y, dot, false);
-                    dot = __builtin_amdgcn_udot4(qv.z, bv[u].z, dot, false);
-                    dot = __builtin_amdgcn_udot4(qv.w, bv[u].w, dot, false);
-                    dot += __shfl_xor(dot, 1);
-                    dot += __shfl_xor(dot, 2);
-                    dot += __shfl_xor(dot, 4);
-                    const uint32_t d = nq + (uint32_t)bn[u] - 2u * dot;
-                    const uint32_t j = jv[u];
-                    if (j != 0xFFFFFFFFu && j != i1 && j != i2) {
-                        const bool c1 = lex_less(d, j, m1, i1), c2 = lex_less(d, j, m2, i2);
-                        m2 = c1 ? m1 : (c2 ? d : m2);
-                        i2 = c1 ? i1 : (c2 ? j : i2);
-                        m1 = c1 ? d : m1;
-                        i1 = c1 ? j : i1;
-                    }
-                }
-                p0 = pn;
-#pragma unroll
-                for (int u = 0; u < UNR; u++) jv[u] = jn[u];
-            }
-        }
-    }
-    // all 8 lanes of a slot hold the same list; merge the 8 slots
+    return nq;
+}
+
+// The 8 slots of a wave hold disjoint or overlapping lists: merge them.
+__device__ inline void merge_slots(uint32_t& m1, uint32_t& i1, uint32_t& m2, uint32_t& i2) {
 #pragma unroll
     for (int off = 8; off < 64; off <<= 1) {
         const uint32_t b1 = __shfl_xor(m1, off), ib1 = __shfl_xor(i1, off);
         const uint32_t b2 = __shfl_xor(m2, off), ib2 = __shfl_xor(i2, off);
         merge2_dedup(m1, i1, m2, i2, b1, ib1, b2, ib2);
     }
-    if (lane == 0) {
-        const bool h1 = i1 != 0xFFFFFFFFu, h2 = i2 != 0xFFFFFFFFu;
-        const float d1 = h1 ? (float)m1 : 3.402823466e+38f, d2 = h2 ? (float)m2 : 3.402823466e+38f;
-        if (a.idx) {
-            a.idx[2 * (size_t)q] = i1;
-            a.idx[2 * (size_t)q + 1] = i2;
-        }
-        if (a.dist) {
-            a.dist[2 * (size_t)q] = d1;
-            a.dist[2 * (size_t)q + 1] = d2;
-        }
-        if (a.ncand) a.ncand[q] = nc;
-        if (a.matches) a.matches[q] = (h1 && sqrtf(d1) / sqrtf(d2) < a.ratio) ? i1 : 0xFFFFFFFFu;
+}
+
+__device__ inline void write_result(const RerankArgs& a, uint32_t q, uint32_t m1, uint32_t i1, uint32_t m2,
+                                    uint32_t i2, uint32_t nc) {
+    const bool h1 = i1 != 0xFFFFFFFFu, h2 = i2 != 0xFFFFFFFFu;
+    const float d1 = h1 ? (float)m1 : 3.402823466e+38f, d2 = h2 ? (float)m2 : 3.402823466e+38f;
+    if (a.idx) {
+        a.idx[2 * (size_t)q] = i1;
+        a.idx[2 * (size_t)q + 1] = i2;
     }
+    if (a.dist) {
+        a.dist[2 * (size_t)q] = d1;
+        a.dist[2 * (size_t)q + 1] = d2;
+    }
+    if (a.ncand) a.ncand[q] = nc;
+    if (a.matches) a.matches[q] = (h1 && sqrtf(d1) / sqrtf(d2) < a.ratio) ? i1 : 0xFFFFFFFFu;
+}
+
+// One wave per query: every (table, probe) bucket in turn.  Bucket ranges are
+// looked up for up to 64 (table, probe) pairs at once, one per lane.
+__global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= a.q_n) return;  // whole wave
+    const int g = lane >> 3, s = lane & 7;  // candidate slot, 16-byte segment
+    uint4 qv;
+    const uint32_t nq = load_query(a, q, s, qv);
+    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
+    uint32_t nc = 0;
+    const int TP = a.L * (a.P + 1);
+    for (int tb = 0; tb < TP; tb += 64) {
+        uint32_t rlo = 0, rhi = 0;
+        if (tb + lane < TP) bucket_range(a, tb + lane, q, rlo, rhi);
+        const int nt = min(64, TP - tb);
+        for (int i = 0; i < nt; i++) {
+            const uint32_t lo = __shfl(rlo, i), hi = __shfl(rhi, i);
+            nc += hi - lo;
+            scan_bucket(a, qv, nq, lo, hi, g, s, m1, i1, m2, i2);
+        }
+    }
+    merge_slots(m1, i1, m2, i2);
+    if (lane == 0) write_result(a, q, m1, i1, m2, i2, nc);
+}
+
+// Bucket-ordered variant: a unit is one (table, probe, query) lookup, u =
+// tp * q_n + q; units are visited in order of their bucket (sorted keys
+// (table << b) | code), so the waves that read one bucket's rows run side by
+// side and share them in cache.  Each unit leaves its top-2 and candidate
+// count; lsh_merge_units combines a query's units.
+__global__ void unit_keys(const uint32_t* __restrict__ qcodes, uint32_t nu, uint32_t q_n, int P1, int b,
+                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint32_t t = (u / q_n) / (uint32_t)P1;
+    keys[u] = (b < 32 ? (t << b) : 0u) | qcodes[u];
+    vals[u] = u;
+}
+
+__global__ __launch_bounds__(256) void lsh_rerank_units(RerankArgs a, const uint32_t* __restrict__ order,
+                                                        uint32_t nu, uint4* __restrict__ part,
+                                                        uint32_t* __restrict__ pcnt) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= nu) return;  // whole wave
+    const uint32_t u = order[i];
+    const uint32_t tp = u / a.q_n, q = u - tp * a.q_n;
+    const int g = lane >> 3, s = lane & 7;
+    uint4 qv;
+    const uint32_t nq = load_query(a, q, s, qv);
+    uint32_t lo, hi;
+    bucket_range(a, (int)tp, q, lo, hi);
+    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
+    scan_bucket(a, qv, nq, lo, hi, g, s, m1, i1, m2, i2);
+    merge_slots(m1, i1, m2, i2);
+    if (lane == 0) {
+        part[u] = make_uint4(m1, i1, m2, i2);
+        pcnt[u] = hi - lo;
+    }
+}
+
+__global__ void lsh_merge_units(RerankArgs a, const uint4* __restrict__ part, const uint32_t* __restrict__ pcnt) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.q_n) return;
+    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
+    uint32_t nc = 0;
+    const int TP = a.L * (a.P + 1);
+    for (int tp = 0; tp < TP; tp++) {
+        const uint4 p = part[(size_t)tp * a.q_n + q];
+        merge2_dedup(m1, i1, m2, i2, p.x, p.y, p.z, p.w);
+        nc += pcnt[(size_t)tp * a.q_n + q];
+    }
+    write_result(a, q, m1, i1, m2, i2, nc);
 }
 
 // ---------------------------------------------------------- hyperplanes --
@@ -436,6 +515,42 @@ int query_codes(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, uint3
     return SIFT_OK;
 }
 
+// Bucket-ordered re-rank (lsh_rerank_units) when the lookups are expected to
+// bring many candidates each (at least 32 per (table, probe) lookup and 4M in
+// all): the unit sort and merge then cost little against reading each
+// bucket's rows once per group of waves instead of once per query.  Small
+// buckets go one wave per query (lsh_rerank).  SIFT_LSH_ORDER=0/1 forces
+// either.
+bool bucket_ordered(const sift_lsh_index* ix, uint32_t q_n) {
+    if (const char* e = std::getenv("SIFT_LSH_ORDER")) return e[0] == '1';
+    const double lookups = (double)q_n * ix->L * (ix->probes + 1);
+    return ix->cand_per_lookup >= 32.0 && ix->cand_per_lookup * lookups >= 4e6;
+}
+
+int rerank_units(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, const RerankArgs& a) {
+    const uint32_t TP = (uint32_t)(ix->L * (ix->probes + 1));
+    const uint32_t nu = TP * a.q_n;
+    for (DevBuf* b : {&sc->ukeys, &sc->uvals, &sc->utk, &sc->utv, &sc->ucnt}) SIFT_TRY(reserve(h, *b, (size_t)nu * 4));
+    SIFT_TRY(reserve(h, sc->upart, (size_t)nu * 16));
+    auto* k = static_cast<uint32_t*>(sc->ukeys.p);
+    auto* v = static_cast<uint32_t*>(sc->uvals.p);
+    auto* tk = static_cast<uint32_t*>(sc->utk.p);
+    auto* tv = static_cast<uint32_t*>(sc->utv.p);
+    unit_keys<<<(nu + 255) / 256, 256, 0, h->stream>>>(a.qcodes, nu, a.q_n, ix->probes + 1, ix->nbits, k, v);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    int tbits = 0;
+    while ((1 << tbits) < ix->L) tbits++;
+    const int bits = std::min(32, ix->nbits + tbits);
+    SIFT_TRY(radix_sort_pairs(h, &k, &v, &tk, &tv, nu, bits, sc->uhist, sc->ulevels));
+    auto* part = static_cast<uint4*>(sc->upart.p);
+    auto* cnt = static_cast<uint32_t*>(sc->ucnt.p);
+    lsh_rerank_units<<<(nu + 3) / 4, 256, 0, h->stream>>>(a, v, nu, part, cnt);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    lsh_merge_units<<<(a.q_n + 255) / 256, 256, 0, h->stream>>>(a, part, cnt);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
 void free_index(sift_lsh_index* ix) {
     if (ix->owns_mlps)
         for (sift_mlp* m : ix->mlps) sift_mlp_destroy(m);
@@ -477,6 +592,22 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
     offs[i] = lo;
 }
 
+// Sum over sorted positions of the size of the bucket holding them (one
+// 64-bit atomic per wave).
+__global__ void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n, uint32_t L,
+                          unsigned long long* __restrict__ acc) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long s = 0;
+    if (p < (size_t)L * n) {
+        const uint32_t t0 = (uint32_t)(p / n) * n, t1 = t0 + n;
+        const uint32_t k = keys[p];
+        s = upper_bound_u32(keys, (uint32_t)p, t1, k) - lower_bound_u32(keys, t0, (uint32_t)p, k);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+}
+
 // The hyperplanes of every table as MFMA B operands: 32 rows per table,
 // planes [nbits, 32) zero.
 int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
@@ -498,13 +629,31 @@ int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
     return rc;
 }
 
+// Bucket starts (b <= OFFS_MAX_BITS) and the mean candidates per lookup of a
+// point drawn like the base rows: sum over buckets of size^2 / (L n), i.e.
+// the mean over sorted positions of the size of the bucket holding them.
 int build_offsets(sift_context* h, sift_lsh_index* ix) {
-    if (ix->nbits > OFFS_MAX_BITS || ix->n == 0) return SIFT_OK;
-    const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
-    SIFT_TRY(reserve(h, ix->offs, no * 4));
-    bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
-        static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L, static_cast<uint32_t*>(ix->offs.p));
+    ix->cand_per_lookup = 0.0;
+    if (ix->n == 0) return SIFT_OK;
+    if (ix->nbits <= OFFS_MAX_BITS) {
+        const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
+        SIFT_TRY(reserve(h, ix->offs, no * 4));
+        bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
+            static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L,
+            static_cast<uint32_t*>(ix->offs.p));
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
+    SIFT_TRY(reserve(h, ix->hist, 8));
+    auto* acc = static_cast<unsigned long long*>(ix->hist.p);
+    SIFT_HIP_TRY(h, hipMemsetAsync(acc, 0, 8, h->stream));
+    const size_t total = (size_t)ix->L * ix->n;
+    bucket_sq<<<(unsigned)((total + 255) / 256), 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n,
+                                                                      (uint32_t)ix->L, acc);
     SIFT_HIP_TRY(h, hipGetLastError());
+    unsigned long long sum = 0;
+    SIFT_HIP_TRY(h, hipMemcpyAsync(&sum, acc, 8, hipMemcpyDeviceToHost, h->stream));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    ix->cand_per_lookup = (double)sum / (double)total;
     return SIFT_OK;
 }
 
@@ -567,8 +716,10 @@ void sift_lsh_state_free(sift_context* h) {
     auto* s = static_cast<LshScratch*>(h->lsh_state);
     for (DevBuf* b : {&s->q_u8, &s->q_nrm, &s->qcodes, &s->qbase, &s->qmarg, &s->flag, &s->q_tiles.tiles, &s->q_tiles.norm,
                       &s->q_tiles.norm_lane, &s->fb_nc, &s->fb_dist, &s->fb_list, &s->fb_count, &s->fb_map, &s->fb_out,
-                      &s->fb_q.tiles, &s->fb_q.norm, &s->fb_q.norm_lane})
+                      &s->fb_q.tiles, &s->fb_q.norm, &s->fb_q.norm_lane, &s->ukeys, &s->uvals, &s->utk, &s->utv,
+                      &s->uhist, &s->upart, &s->ucnt})
         release(*b);
+    for (auto& l : s->ulevels) release(l);
     delete s;
     h->lsh_state = nullptr;
 }
@@ -669,8 +820,12 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     a.ratio = ratio;
     {
         ProfScope ps(h, PROF_LSH_QUERY);
-        lsh_rerank<<<(q_n + 3) / 4, 256, 0, h->stream>>>(a);
-        SIFT_HIP_TRY(h, hipGetLastError());
+        if (bucket_ordered(ix, q_n)) {
+            SIFT_TRY(rerank_units(h, ix, sc, a));
+        } else {
+            lsh_rerank<<<(q_n + 3) / 4, 256, 0, h->stream>>>(a);
+            SIFT_HIP_TRY(h, hipGetLastError());
+        }
     }
     if (blocking) SIFT_TRY(check_flag(h, flag, "sift_lsh_query"));
     return SIFT_OK;

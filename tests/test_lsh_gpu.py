@@ -184,3 +184,26 @@ def test_lsh_multiprobe_matches_oracle(handle, nbits, L, P):
     with pytest.raises(SiftError):
         ix.set_probes(min(nbits, 16) + 1)
     ix.close()
+
+
+@pytest.mark.parametrize("nbits,L,P", [(8, 2, 0), (12, 5, 12), (16, 3, 2), (24, 5, 13), (32, 1, 0)])
+def test_lsh_bucket_ordered_rerank(handle, monkeypatch, nbits, L, P):
+    """The bucket-ordered re-rank (one wave per (table, probe, query) lookup in
+    bucket order, then a per-query merge that drops rows found twice) gives
+    the per-query re-rank's outputs exactly, and so the oracle's."""
+    b = datagen.base(12_000, seed=41)
+    q = datagen.queries(b, 500, seed=42)
+    ix = LSHIndex(handle, b, nbits, L, seed=11)
+    ix.set_probes(P)
+    say("lsh order", nbits, L, P)
+    monkeypatch.setenv("SIFT_LSH_ORDER", "0")
+    want = ix.query(q)
+    monkeypatch.setenv("SIFT_LSH_ORDER", "1")
+    got = ix.query(q)
+    for w, g in zip(want, got):
+        np.testing.assert_array_equal(g, w)
+    if P == 0:
+        cq, cb = ix.codes(q), ix.codes(b)
+        oi, _, _ = oracle.lsh_query(q, b, cq, cb)
+        np.testing.assert_array_equal(got[0], oi)
+    ix.close()
