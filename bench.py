@@ -96,7 +96,11 @@ def lsh_sweep(args, ms, q, b, h, qd, bd):
                 "build_s": build_s, "rerank_ms": rr_s * 1e3,
                 "roofline": {"bound": "hbm", "achieved": bytes_q * M / rr_s / 1e9, "peak": 8000.0,
                              "unit": "GB/s", "frac": bytes_q * M / rr_s / 8e12,
-                             "bytes_per_query": bytes_q}}), flush=True)
+                             "bytes_per_query": bytes_q,
+                             "note": "candidate bytes, each row counted once per query that reads it; "
+                                     "the bucket-ordered re-rank serves a bucket's rows from cache to "
+                                     "the queries sharing it, so frac above 1 means the re-rank is no "
+                                     "longer bound by HBM"}}), flush=True)
             ix.close()
 
 
