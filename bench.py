@@ -194,7 +194,8 @@ def pairs_bench(args, ms, h, compute):
                                "launch per step in a HIP graph" % (args.pairs, args.compute),
                    "pairs": args.pairs, "queries": Q, "base_rows": int(nr.sum()),
                    "shape": info[3:6], "work_items": info[1], "grid": info[2]},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TOP/s" if args.compute == "i8" else "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel": "pairs_top2",
                      "avg_launch_ms": k_avg * 1e3, "flops_per_launch": flops},
         "cpu_baseline": cpu,
@@ -552,7 +553,8 @@ def main():
         "config": {"workload": "exact 2-NN brute force, base N=%d, queries M=%d, d=128, "
                                "float32 inputs, %s MFMA, fused distance+top-2" % (N, M, args.compute),
                    "n_base": N, "n_query": M, "dim": 128, "compute": args.compute},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TOP/s" if args.compute == "i8" else "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
                      "kernel": "fused_top2", "kernel_ms_per_step": fused_step_s * 1e3,
                      "launches_per_step": fused_n / args.steps, "flops_per_step": flops},
