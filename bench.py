@@ -369,7 +369,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--compute", default="bf16", choices=list(PEAK))
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--m", type=int, default=10_000)

@@ -314,9 +314,11 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
 
 // Bases of at least this many tiles pick their launch shape by timing: the
 // first call of a kind on a handle runs each of the kind's first two listed
-// shapes once (every shape gives identical results) and keeps the faster.
+// shapes TUNE_ROUNDS times, alternating (every shape gives identical
+// results), and keeps the one with the fastest run.
 // The best shape differs between boxes by up to 10% with no static rule.
 constexpr uint32_t TUNE_MIN_TILES = 8192;
+constexpr int TUNE_ROUNDS = 3;
 
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
                 uint32_t r_n, const Knn2Out& out) {
@@ -337,23 +339,27 @@ int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint3
         (void)hipEventDestroy(e[0]);
         return set_error(h, SIFT_ERR_HIP, "hipEventCreate");
     }
+    // TUNE_ROUNDS interleaved rounds, fastest run per shape: a cold GPU's
+    // first launches run slow, so one run each would favour the later shape
     int best = 0, rc = SIFT_OK;
-    float best_ms = 3.4e38f;
-    for (int i = 0; i < nc && rc == SIFT_OK; i++) {
-        (void)hipEventRecord(e[0], h->stream);
-        rc = launch_fused_shape(h, *cand[i], qs, bs, q_n, r_n, out);
-        (void)hipEventRecord(e[1], h->stream);
-        if (rc != SIFT_OK) break;
-        float ms = 0.f;
-        if (hipEventSynchronize(e[1]) != hipSuccess || hipEventElapsedTime(&ms, e[0], e[1]) != hipSuccess) {
-            rc = set_error(h, SIFT_ERR_HIP, "shape timing");
-            break;
-        }
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = i;
+    float best_ms[16];
+    for (int i = 0; i < nc; i++) best_ms[i] = 3.4e38f;
+    for (int r = 0; r < TUNE_ROUNDS && rc == SIFT_OK; r++) {
+        for (int i = 0; i < nc && rc == SIFT_OK; i++) {
+            (void)hipEventRecord(e[0], h->stream);
+            rc = launch_fused_shape(h, *cand[i], qs, bs, q_n, r_n, out);
+            (void)hipEventRecord(e[1], h->stream);
+            if (rc != SIFT_OK) break;
+            float ms = 0.f;
+            if (hipEventSynchronize(e[1]) != hipSuccess || hipEventElapsedTime(&ms, e[0], e[1]) != hipSuccess) {
+                rc = set_error(h, SIFT_ERR_HIP, "shape timing");
+                break;
+            }
+            best_ms[i] = std::min(best_ms[i], ms);
         }
     }
+    for (int i = 1; i < nc; i++)
+        if (best_ms[i] < best_ms[best]) best = i;
     (void)hipEventDestroy(e[0]);
     (void)hipEventDestroy(e[1]);
     if (rc == SIFT_OK) h->tuned_shape[kind] = best;  // the last launch already wrote the outputs
