@@ -522,8 +522,9 @@ int query_codes(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, uint3
 // buckets go one wave per query (lsh_rerank).  SIFT_LSH_ORDER=0/1 forces
 // either.
 bool bucket_ordered(const sift_lsh_index* ix, uint32_t q_n) {
-    if (const char* e = std::getenv("SIFT_LSH_ORDER")) return e[0] == '1';
     const double lookups = (double)q_n * ix->L * (ix->probes + 1);
+    if (lookups >= 2147483648.0) return false;  // unit ids are 32-bit
+    if (const char* e = std::getenv("SIFT_LSH_ORDER")) return e[0] == '1';
     return ix->cand_per_lookup >= 32.0 && ix->cand_per_lookup * lookups >= 4e6;
 }
 
