@@ -229,7 +229,8 @@ def learned_bench(args, ms, h):
     qd, bd, td = ms.DeviceArray.from_numpy(q), ms.DeviceArray.from_numpy(b), ms.DeviceArray.from_numpy(xtr)
     ex = ms.knn2(h, qd, bd, compute=ms.COMPUTE_BF16, ratio=0.8)
     exact, exact_m = ex[0], ex[3]
-    bf = ms.BruteForceIndex(h, bd, compute=ms.COMPUTE_BF16, type_=ms.FLOAT_DEVICE)
+    # the exact fallback and baseline: the fastest exact kind for these integers (int8 MFMA)
+    bf = ms.BruteForceIndex(h, bd, compute=ms.COMPUTE_I8, type_=ms.FLOAT_DEVICE)
     bo = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32), None, None)
     for _ in range(args.warmup):
         bf.query(qd, outputs=bo, type_=ms.FLOAT_DEVICE, blocking=False)

@@ -53,7 +53,7 @@ enum {
 
 /* Arithmetic used by the fused distance + top-2 kernel. */
 enum {
-    SIFT_COMPUTE_AUTO = 0, /* integer data in [0,255] -> BF16, else FP32     */
+    SIFT_COMPUTE_AUTO = 0, /* integer data in [0,255] -> I8, else FP32       */
     SIFT_COMPUTE_BF16 = 1, /* BF16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F16 = 2,  /* FP16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F32 = 3,  /* FP32 MFMA                                      */
@@ -317,7 +317,7 @@ int sift_pairs_plan_set_shape(sift_pairs_plan_t plan, int qw, int waves, int tps
  * results of BF16/F16/I8 plans are then not exact). */
 int sift_pairs_plan_info(sift_pairs_plan_t plan, int64_t* out9);
 int sift_pairs_plan_destroy(sift_pairs_plan_t plan);
-/* One-shot: plan (AUTO allowed: BF16 for integer data in [0,255], else F32),
+/* One-shot: plan (AUTO allowed: I8 for integer data in [0,255], else F32),
  * execute, synchronize, destroy.  type 0 (host float) is staged.  Explicit
  * BF16/F16/I8 with non-integer data returns SIFT_ERR_NOT_EXACT. */
 int sift_match_pairs(sift_handle_t h, const void* q_desc, const void* r_desc, int type,

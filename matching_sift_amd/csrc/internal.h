@@ -124,7 +124,7 @@ uint32_t query_rows_pad(int kind, uint32_t q_n);
 
 bool valid_type(int type);
 // Resolve SIFT_COMPUTE_* to a packed kind.  AUTO: integer data in [0,255] ->
-// BF16 (exact), otherwise `fallback`.  AUTO reads one device flag (syncs).
+// I8 (exact), otherwise `fallback`.  AUTO reads one device flag (syncs).
 int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32_t q_n,
                  uint32_t r_n, int compute, int fallback, int* kind);
 

@@ -380,8 +380,9 @@ namespace sift {
 bool valid_type(int type) { return type >= SIFT_FLOAT_HOST && type <= SIFT_U8_DEVICE; }
 
 // Resolve SIFT_COMPUTE_* to a packed kind.  AUTO: integer data in [0,255] ->
-// BF16 (exact), otherwise `fallback` (F32 for the FP32 entry points, F16 for
-// the fp16 one).  Reads one device flag (a 4-byte synchronous copy).
+// I8 (exact, and the fastest exact kind: int8 MFMA at twice the BF16 rate),
+// otherwise `fallback` (F32 for the FP32 entry points, F16 for the fp16 one).
+// Reads one device flag (a 4-byte synchronous copy).
 int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32_t q_n,
                  uint32_t r_n, int compute, int fallback, int* kind) {
     switch (compute) {
@@ -394,7 +395,7 @@ int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32
         default: return set_error(h, SIFT_ERR_INVALID, "unknown compute kind " + std::to_string(compute));
     }
     if (type == SIFT_U8_DEVICE) {
-        *kind = KIND_BF16;
+        *kind = KIND_I8;
         return SIFT_OK;
     }
     SIFT_TRY(reserve(h, h->flag, 16));
@@ -405,7 +406,7 @@ int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32
     int bad = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-    *kind = bad ? fallback : KIND_BF16;
+    *kind = bad ? fallback : KIND_I8;
     return SIFT_OK;
 }
 
@@ -475,7 +476,7 @@ int sift_knn2_async(sift_handle_t h, const void* q, const void* r, int type, uin
 }
 
 // src/knn_brute.cu:24-209.  fp16 semantics for non-integer data (the
-// reference converts to half); exact BF16 for integer SIFT data.
+// reference converts to half); exact int8 MFMA for integer SIFT data.
 int cublas_2nn_sift(void* q_points, void* r_points, int type, uint32_t q_n, uint32_t r_n,
                     uint32_t* matches, float threshold, sift_handle_t handle, int stream_n,
                     size_t use, int warps) {
