@@ -268,6 +268,12 @@ int sift_mlp_train(sift_handle_t h, sift_mlp_t mlp, const void* x, uint32_t n, c
  * confident bits flipped (smallest |h.x| or |z| first; 0 = off, at most
  * min(nbits, 16)).  A setting of the index used by every later query. */
 int sift_lsh_set_probes(sift_lsh_index_t index, int probes);
+/* Mean bucket size seen by a point drawn like the base rows (no counterpart
+ * in the reference): sum over the buckets of every table of size^2, divided
+ * by L * n, i.e. the expected candidates of one (table, probe) lookup.  The
+ * query picks its re-rank schedule from it; callers can size probes and
+ * tables with it. */
+int sift_lsh_bucket_stat(sift_lsh_index_t index, double* cand_per_lookup);
 /* The codes a query looks up, device [L][probes + 1][q_n]: per table the
  * base code, then the probes in order. */
 int sift_lsh_probe_codes(sift_handle_t h, sift_lsh_index_t index, const void* q_points, int type,

@@ -73,6 +73,7 @@ SIGNATURES = {
     "sift_mlp_train": (i32, [vp, vp, vp, u32, vp, u32, i32, vp, vp]),
     "sift_lsh_build_learned": (i32, [vp, vp, i32, u32, vp, i32, C.POINTER(vp)]),
     "sift_lsh_set_probes": (i32, [vp, i32]),
+    "sift_lsh_bucket_stat": (i32, [vp, C.POINTER(C.c_double)]),
     "sift_lsh_probe_codes": (i32, [vp, vp, vp, i32, u32, vp]),
     "sift_lsh_save": (i32, [vp, C.c_char_p]),
     "sift_lsh_load": (i32, [vp, C.c_char_p, C.POINTER(vp)]),

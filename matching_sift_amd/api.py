@@ -205,6 +205,13 @@ class LSHIndex:
         check(lib().sift_lsh_set_probes(self.ix, int(probes)), self.handle.h)
         self.probes = int(probes)
 
+    def bucket_stat(self):
+        """Mean bucket size seen by a base-like point: sum of size^2 over the
+        buckets of every table / (L n), the expected candidates per lookup."""
+        v = C.c_double()
+        check(lib().sift_lsh_bucket_stat(self.ix, C.byref(v)), self.handle.h)
+        return v.value
+
     def probe_codes(self, q, type_=None):
         """[L][probes + 1][q] codes looked up per query (base code first)."""
         qd = to_device(q)

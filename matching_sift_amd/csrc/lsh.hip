@@ -593,20 +593,41 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
     offs[i] = lo;
 }
 
-// Sum over sorted positions of the size of the bucket holding them (one
-// 64-bit atomic per wave).
-__global__ void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n, uint32_t L,
-                          unsigned long long* __restrict__ acc) {
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Sum over buckets of size^2 (= sum over sorted positions of the size of the
+// bucket holding them): the thread at each bucket's first position finds
+// the bucket's end by a galloping search (a few loads for small buckets),
+// the rest only compare with their left neighbour.  Grid-stride, one 64-bit
+// atomic per block.
+__global__ __launch_bounds__(256) void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n, uint32_t L,
+                                                 unsigned long long* __restrict__ acc) {
+    __shared__ unsigned long long wsum[4];
+    const size_t total = (size_t)L * n;
     unsigned long long s = 0;
-    if (p < (size_t)L * n) {
-        const uint32_t t0 = (uint32_t)(p / n) * n, t1 = t0 + n;
-        const uint32_t k = keys[p];
-        s = upper_bound_u32(keys, (uint32_t)p, t1, k) - lower_bound_u32(keys, t0, (uint32_t)p, k);
+    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t t0 = (uint32_t)(p / n) * n, t1 = t0 + n, pp = (uint32_t)p;
+        const uint32_t k = keys[pp];
+        if (pp == t0 || keys[pp - 1] != k) {  // first position of its bucket
+            uint32_t lo = pp, hi = t1;  // keys[lo] == k
+            for (uint64_t d = 1;; d <<= 1) {
+                const uint64_t nx = (uint64_t)pp + d;
+                if (nx >= t1 || keys[nx] != k) {
+                    hi = (uint32_t)min<uint64_t>(nx, t1);
+                    break;
+                }
+                lo = (uint32_t)nx;
+            }
+            const unsigned long long len = upper_bound_u32(keys, lo + 1, hi, k) - pp;
+            s += len * len;
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long b = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (b) atomicAdd(acc, b);
+    }
 }
 
 // The hyperplanes of every table as MFMA B operands: 32 rows per table,
@@ -648,8 +669,8 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
     auto* acc = static_cast<unsigned long long*>(ix->hist.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(acc, 0, 8, h->stream));
     const size_t total = (size_t)ix->L * ix->n;
-    bucket_sq<<<(unsigned)((total + 255) / 256), 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n,
-                                                                      (uint32_t)ix->L, acc);
+    const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
+    bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, (uint32_t)ix->L, acc);
     SIFT_HIP_TRY(h, hipGetLastError());
     unsigned long long sum = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&sum, acc, 8, hipMemcpyDeviceToHost, h->stream));
@@ -837,6 +858,12 @@ int sift_lsh_set_probes(sift_lsh_index_t ix, int probes) {
     if (probes < 0 || probes > std::min(ix->nbits, MAX_PROBES))
         return set_error(ix->h, SIFT_ERR_INVALID, "probes must be in [0, min(nbits, 16)]");
     ix->probes = probes;
+    return SIFT_OK;
+}
+
+int sift_lsh_bucket_stat(sift_lsh_index_t ix, double* cand_per_lookup) {
+    if (!ix || !cand_per_lookup) return set_error(ix ? ix->h : nullptr, SIFT_ERR_INVALID, "null index/output");
+    *cand_per_lookup = ix->cand_per_lookup;
     return SIFT_OK;
 }
 

@@ -7,7 +7,8 @@
 //   hist    : per-block 256-bin digit histogram (LDS atomics), bin-major
 //   scan    : exclusive scan of the histogram -> global digit offsets
 //   scatter : stable scatter; ranks among equal digits come from wave-wide
-//             ballots, so elements keep their input order within a digit.
+//             ballots, so elements keep their input order within a digit;
+//             each chunk is ordered by digit in LDS and written run by run.
 #include "internal.h"
 
 namespace sift {
@@ -35,26 +36,50 @@ __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ key
     hist[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// Stable scatter of one 4096-element chunk: ranks come from wave-wide
+// ballots, the chunk is first ordered by digit in LDS, then written out with
+// consecutive threads on consecutive positions of each digit's run (whole
+// runs instead of scattered 4-byte writes).
 __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ kin,
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout, uint32_t n,
                                                    int shift, const uint32_t* __restrict__ offs,
                                                    uint32_t nblk) {
-    __shared__ uint32_t wh[RS_W][256];
+    __shared__ uint32_t sk[RS_CHUNK], sv[RS_CHUNK];
+    __shared__ uint32_t wh[RS_W][256];  // per-wave digit counts, then local cursors
+    __shared__ uint32_t lstart[256], gstart[256], wsum[RS_W];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < RS_W; k++) wh[k][threadIdx.x] = 0;
     __syncthreads();
-    const size_t sub = (size_t)blockIdx.x * RS_CHUNK + (size_t)w * RS_SUB;
+    const size_t base = (size_t)blockIdx.x * RS_CHUNK;
+    const size_t sub = base + (size_t)w * RS_SUB;
     for (int step = 0; step < RS_SUB / 64; step++) {
         const size_t idx = sub + (size_t)step * 64 + lane;
         if (idx < n) atomicAdd(&wh[w][(kin[idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    {  // per digit: global offset of this block, then prefix over waves
+    {  // per digit: start of its run in the chunk (scan over digits), this
+       // block's global offset, and each wave's local cursor
         const int bin = threadIdx.x;
-        uint32_t run = offs[(size_t)bin * nblk + blockIdx.x];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int k = 0; k < RS_W; k++) tot += wh[k][bin];
+        uint32_t x = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[w] = x;
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int k = 0; k < w; k++) pre += wsum[k];
+        const uint32_t ls = pre + x - tot;
+        lstart[bin] = ls;
+        gstart[bin] = offs[(size_t)bin * nblk + blockIdx.x];
+        uint32_t run = ls;
 #pragma unroll
         for (int k = 0; k < RS_W; k++) {
             const uint32_t c = wh[k][bin];
@@ -80,11 +105,20 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
         const uint32_t pos = wh[w][d];
         if (valid) {
-            kout[pos + rank] = key;
-            vout[pos + rank] = val;
+            sk[pos + rank] = key;
+            sv[pos + rank] = val;
         }
         // lowest lane of each digit group advances that digit's cursor
         if (valid && (peers & lt) == 0) wh[w][d] = pos + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    const uint32_t m = (uint32_t)min<size_t>((size_t)RS_CHUNK, n - base);
+    for (uint32_t i = threadIdx.x; i < m; i += RS_T) {
+        const uint32_t key = sk[i];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t pos = gstart[d] + (i - lstart[d]);
+        kout[pos] = key;
+        vout[pos] = sv[i];
     }
 }
 

@@ -207,3 +207,17 @@ def test_lsh_bucket_ordered_rerank(handle, monkeypatch, nbits, L, P):
         oi, _, _ = oracle.lsh_query(q, b, cq, cb)
         np.testing.assert_array_equal(got[0], oi)
     ix.close()
+
+
+@pytest.mark.parametrize("nbits,L", [(4, 3), (12, 2), (24, 2), (32, 1)])
+def test_lsh_bucket_stat(handle, nbits, L):
+    """The index's mean bucket size seen by a base row equals the sum of
+    squared bucket sizes over the tables / (L n), counted from the codes."""
+    b = datagen.base(9_000, seed=51)
+    ix = LSHIndex(handle, b, nbits, L, seed=13)
+    say("lsh bucket stat", nbits, L)
+    cb = ix.codes(b)
+    want = sum(float((np.unique(cb[t], return_counts=True)[1].astype(np.float64) ** 2).sum())
+               for t in range(L)) / (L * len(b))
+    assert ix.bucket_stat() == pytest.approx(want, rel=1e-12)
+    ix.close()
