@@ -313,3 +313,24 @@ def test_fp8_rejects_non_integer(handle):
     with pytest.raises(SiftError) as e:
         knn2(handle, q, q, compute=COMPUTE_FP8)
     assert e.value.code == -5
+
+
+def test_random_sizes_every_exact_kind(handle):
+    """Seeded random (N, M) in the ragged and tile-boundary sizes, SIFT-like or
+    tie-heavy (values 0/85/170/255) data: BF16, FP16, I8, FP32 and AUTO all
+    equal host_brute exactly."""
+    rng = np.random.default_rng(2024)
+    for it in range(14):
+        N = int(rng.choice([1, 2, 31, 32, 33, 95, 1000, 4097, 20000, 70000]))
+        M = int(rng.choice([1, 2, 63, 64, 65, 257, 1000]))
+        if it % 2:
+            r = rng.integers(0, 4, size=(N, 128)).astype(np.float32) * 85
+            q = rng.integers(0, 4, size=(M, 128)).astype(np.float32) * 85
+        else:
+            r = datagen.base(N, seed=100 + it)
+            q = datagen.queries(r, M, seed=200 + it)
+        want = oracle.host_brute(q, r, threads=8)
+        for kind in (COMPUTE_BF16, COMPUTE_F16, COMPUTE_I8, COMPUTE_F32, COMPUTE_AUTO):
+            say("random sizes", it, N, M, kind)
+            got = knn2(handle, q, r, compute=kind)[2]
+            np.testing.assert_array_equal(got, want)
