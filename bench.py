@@ -40,6 +40,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """CPU model and logical core count of the host the baseline runs on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_logical_cores": os.cpu_count()}
+
+
 def cpu_baseline(q, b, dev_idx, batch, threads, seconds):
     """The oracle's host_brute over batches of `batch` queries against the
     full base until `seconds` have passed (at least one batch)."""
@@ -181,7 +195,7 @@ def pairs_bench(args, ms, h, compute):
         dt = time.perf_counter() - t1
         cpu = {"value": done / dt, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d of %d pairs (%d queries), host_brute restatement (oracle/) + "
-                         "ratio test, %.1f s" % (p, args.pairs, done, dt)}
+                         "ratio test, %.1f s" % (p, args.pairs, done, dt), **host_info()}
     out = {
         "metric": "batched image-pair 2-NN + ratio-test queries/sec",
         "value": Q * args.steps / elapsed, "unit": "queries/s", "n_gpus": 1,
@@ -533,7 +547,7 @@ def main():
         qps_cpu, dt, ok, done = cpu_baseline(q, b, check_idx, args.cpu_queries, threads, args.cpu_seconds)
         cpu = {"value": qps_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d queries x full %d base, host_brute restatement (oracle/), "
-                         "%.1f s; indices equal to the GPU's exact result: %s" % (done, N, dt, ok)}
+                         "%.1f s; indices equal to the GPU's exact result: %s" % (done, N, dt, ok), **host_info()}
 
     out = {
         "metric": ("2-NN queries/sec at 1M x 128 base (exact, bit-exact vs host_brute)"
