@@ -82,9 +82,17 @@ def lsh_sweep(args, ms, q, b, h, qd, bd):
     for nb in bits:
         for L in tables:
             h.sync()
+            h.profile(True)
+            h.profile_reset()
             t0 = time.perf_counter()
             ix = ms.LSHIndex(h, bd, nb, L, seed=1000 + nb, type_=ms.FLOAT_DEVICE)
             build_s = time.perf_counter() - t0
+            build_k = {k: h.profile_read(k)[0] for k in ("pack", "lsh_proj", "lsh_sort")}
+            h.profile(False)
+            # index build, algorithmic HBM bytes: float32 rows in, codes out, one read and
+            # one write of every (key, row) pair, bucket offsets
+            build_bytes = N * 128 * 4 + N * L * 4 + 2 * N * L * 8 + (L * ((1 << nb) + 1) * 4 if nb <= 20 else 0)
+            build_gpu_ms = sum(build_k.values())
             for _ in range(args.warmup):
                 ix.query(qd, outputs=outs, type_=ms.FLOAT_DEVICE, blocking=False)
             h.sync()
@@ -108,6 +116,10 @@ def lsh_sweep(args, ms, q, b, h, qd, bd):
                 "nbits": nb, "tables": L, "value": M / dt, "unit": "queries/s",
                 "recall_nn1": r1, "recall_2nn": r2, "mean_candidates": cbar,
                 "build_s": build_s, "rerank_ms": rr_s * 1e3,
+                "build_gpu_ms": build_gpu_ms, "build_kernels_ms": build_k,
+                "build_roofline": {"bound": "hbm", "bytes": build_bytes,
+                                   "achieved": build_bytes / (build_gpu_ms * 1e-3) / 1e9, "peak": 8000.0,
+                                   "unit": "GB/s", "frac": build_bytes / (build_gpu_ms * 1e-3) / 8e12},
                 "roofline": {"bound": "hbm", "achieved": bytes_q * M / rr_s / 1e9, "peak": 8000.0,
                              "unit": "GB/s", "frac": bytes_q * M / rr_s / 8e12,
                              "bytes_per_query": bytes_q,

@@ -657,21 +657,24 @@ int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
 int build_offsets(sift_context* h, sift_lsh_index* ix) {
     ix->cand_per_lookup = 0.0;
     if (ix->n == 0) return SIFT_OK;
-    if (ix->nbits <= OFFS_MAX_BITS) {
-        const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
-        SIFT_TRY(reserve(h, ix->offs, no * 4));
-        bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
-            static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L,
-            static_cast<uint32_t*>(ix->offs.p));
-        SIFT_HIP_TRY(h, hipGetLastError());
-    }
     SIFT_TRY(reserve(h, ix->hist, 8));
     auto* acc = static_cast<unsigned long long*>(ix->hist.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(acc, 0, 8, h->stream));
     const size_t total = (size_t)ix->L * ix->n;
-    const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
-    bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, (uint32_t)ix->L, acc);
-    SIFT_HIP_TRY(h, hipGetLastError());
+    {
+        ProfScope ps(h, PROF_LSH_SORT);  // bucket structures count with the sort
+        if (ix->nbits <= OFFS_MAX_BITS) {
+            const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
+            SIFT_TRY(reserve(h, ix->offs, no * 4));
+            bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
+                static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L,
+                static_cast<uint32_t*>(ix->offs.p));
+            SIFT_HIP_TRY(h, hipGetLastError());
+        }
+        const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
+        bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, (uint32_t)ix->L, acc);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     unsigned long long sum = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&sum, acc, 8, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -690,7 +693,10 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
     int rc = SIFT_OK;
     do {
         // gather copy + exactness check
-        if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
+        {
+            ProfScope ps(h, PROF_PACK);
+            if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
+        }
         const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
         if ((rc = pack_rows(h, r, type, r_n, rows_pad, KIND_BF16, PACK_PLAIN, h->b_set, flag)) != SIFT_OK) break;
         if ((rc = check_flag(h, flag, "sift_lsh_build")) != SIFT_OK) break;
