@@ -560,6 +560,12 @@ def main():
         cpu = {"value": qps_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d queries x full %d base, host_brute restatement (oracle/), "
                          "%.1f s; indices equal to the GPU's exact result: %s" % (done, N, dt, ok), **host_info()}
+        # the reference's host_brute as written: one thread
+        log("cpu baseline, one thread: batches of 8 queries for ~%.0f s" % (args.cpu_seconds / 3))
+        qps_1, dt_1, ok_1, done_1 = cpu_baseline(q, b, check_idx, 8, 1, args.cpu_seconds / 3)
+        cpu["single_thread"] = {"value": qps_1, "unit": "queries/s", "cores": 1,
+                                "sample": "first %d queries x full %d base, %.1f s; indices equal: %s"
+                                          % (done_1, N, dt_1, ok_1)}
 
     out = {
         "metric": ("2-NN queries/sec at 1M x 128 base (exact, bit-exact vs host_brute)"
