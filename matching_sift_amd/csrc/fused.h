@@ -124,6 +124,10 @@ __device__ inline float next_up(float x) {
     return __uint_as_float(x > 0.f ? u + 1u : u - 1u);
 }
 __device__ inline int next_up(int x) { return x == 0x7FFFFFFF ? x : x + 1; }
+// Threshold of the fast test: float kinds test the key itself; I8 tests
+// floor(key / 2) (its raw accumulator) against ceil(thr / 2).
+__device__ inline float half_up(float x) { return x; }
+__device__ inline int half_up(int x) { return (x >> 1) + (x & 1); }
 
 template <typename T>
 __device__ inline bool lex_less(T x, uint32_t ix, T y, uint32_t iy) {
@@ -158,7 +162,9 @@ struct PairTask {
 struct FusedArgs {
     const uint4* q_tiles;   // packed queries, [qblocks][G][64]
     const uint4* b_tiles;   // packed base, [n_tiles][G][64]
-    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32] (I8 only)
+    const float* b_nl;      // base norms in accumulator-lane order, [n_tiles][32] (I8: int32;
+                            // the brute force passes c >> 1, the pairs kernel c)
+    const int* b_nlc;       // I8 brute force: c in accumulator-lane order (exact keys in the slow path)
     uint4* part;            // [n_slabs][q_stride] {key1, idx1, key2, idx2}
     int* bound;             // per-query inclusive upper bound on the 2nd-best key
                             // (Key::ord form), shared by all blocks; or null
@@ -175,7 +181,7 @@ struct FusedArgs {
     float* out_dist;
     uint32_t* out_matches;
     float ratio;
-    int neg64;  // -64, passed at run time so I8 keys take one v_mad_i32_i24 (brute force: -64 / 32)
+    int neg64;  // pairs kernel, I8: 64, passed at run time so a composite takes one v_mad_i32_i24
     uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
 };
 

@@ -192,9 +192,6 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     __shared__ __align__(16) float s_nl[3][WAVES][NLC * 64];
 
     const int t_begin = task.t_begin, t_end = task.t_end;
-    // I8 keys |b'|^2 - 2 q'.b': |q'.b'| < 2^21, so one 24-bit multiply-add;
-    // the factor comes in at run time (a literal -2 becomes shift + subtract)
-    const int neg2 = a.neg64 / 32;
     const int tid = threadIdx.x;
     const int lane = tid & 63, h = lane >> 5;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar address arithmetic
@@ -214,6 +211,10 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     // Both slots start at (next_up(B), none), so every true top-2 row
     // qualifies.
     key_t m1[QW], m2[QW], thr[QW];
+    // I8 tests the raw accumulator q''.b' + (c >> 1), half the key rounded
+    // down (key = 2 q''.b' + c): any key below thr has accumulator below
+    // ceil(thr / 2), so the test on thr_h = ceil(thr / 2) misses no row
+    key_t thr_h[QW];
     uint32_t i1[QW], i2[QW];
     // Packed mode (pairs kernel, integer kinds): each lane keeps its top-2 as
     // int32 "composites" key*32 + slot so one v_min + one v_med3 per row
@@ -231,6 +232,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         // padding queries admit nothing, so they never flag a tile
         m1[qb] = m2[qb] = init;
         thr[qb] = valid ? init : Key<KIND>::lowest();
+        thr_h[qb] = half_up(thr[qb]);
         i1[qb] = i2[qb] = 0xFFFFFFFFu;
         c1[qb] = c2[qb] = CNONE;
     }
@@ -338,13 +340,17 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         // Float kinds: the accumulator starts at the rows' |b|^2 (accumulator
         // lane order) and the queries are packed as -2q, so the MFMA chain
         // ends at t = |b|^2 - 2 q.b (every partial sum is an integer of
-        // magnitude below 2^24 for SIFT data, hence exact).  I8 starts at 0
-        // and t = |b'|^2 - 2 q'.b' afterwards.
+        // magnitude below 2^24 for SIFT data, hence exact).  I8 (brute force)
+        // starts at c >> 1 and ends at q''.b' + (c >> 1) = floor(t / 2) with
+        // t = |b'|^2 - 2 q'.b' = 2 q''.b' + c; the pairs kernel starts at 0.
         auto tile_acc = [&](int tt, acc_t (&acc)[QW]) {
             acc_t init{};
             if constexpr (FLOAT_KEY) {
 #pragma unroll
                 for (int j = 0; j < 16; j++) init[j] = __int_as_float(nlr[tt][j]);
+            } else if constexpr (!PAIRS) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) init[j] = nlr[tt][j];
             }
 #pragma unroll
             for (int qb = 0; qb < QW; qb++) acc[qb] = KTk::mma_at(afr[tt], qop[qb], 0, init);
@@ -361,10 +367,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             for (int qb = 0; qb < QW; qb++)
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
-                    if constexpr (FLOAT_KEY)
-                        tv[qb][j] = acc[qb][j];
-                    else
-                        tv[qb][j] = __mul24((int)acc[qb][j], neg2) + nlr[tt][j];  // one v_mad_i32_i24
+                    tv[qb][j] = acc[qb][j];  // I8: floor(key / 2), tested against thr_h
                 }
         };
 
@@ -382,7 +385,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 if (tt >= ntile) break;  // uniform: slab end
                 acc_t acc[QW];
                 tile_acc(tt, acc);
-                int w[16];  // I8: |b'|^2 * 32 + 16 + j, so composite = w - 64 q'.b' (one v_mad_i32_i24)
+                int w[16];  // I8: c * 32 + 16 + j, so composite = w + 64 q''.b' (one v_mad_i32_i24)
                 if constexpr (!FLOAT_KEY) {
 #pragma unroll
                     for (int j = 0; j < 16; j++) w[j] = (nlr[tt][j] << 5) | (16 + j);
@@ -443,7 +446,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 const key_t a2 = min3k(v[6], v[7], v[8]), a3 = min3k(v[9], v[10], v[11]);
                 const key_t a4 = min3k(v[12], v[13], v[14]);
                 const key_t mn = min(min3k(a0, a1, a2), min3k(a3, a4, v[15]));
-                const bool hit = __builtin_amdgcn_ballot_w64(mn < thr[qb]) != 0;
+                const bool hit = __builtin_amdgcn_ballot_w64(mn < (FLOAT_KEY ? thr[qb] : thr_h[qb])) != 0;
                 fmask[tt] |= ((hit || last) ? 1u : 0u) << qb;
             }
             if (tt >= ntile) fmask[tt] = 0;
@@ -463,6 +466,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                         for (int j = 0; j < 16; j++) t[j] = tv[tt][qb][j];
                     } else {
+                        // the exact keys: 2 q''.b' + c (the MFMAs again from 0)
                         acc_t acc;
                         acc_t init{};
                         if constexpr (FLOAT_KEY) {
@@ -472,12 +476,19 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                         acc = KTk::mma_at(afr[tt], qop[qb], 0, init);
 #pragma unroll
                         for (int m = 1; m < KTk::NM; m++) acc = KTk::mma_at(afr[tt], qop[qb], m, acc);
+                        if constexpr (FLOAT_KEY) {
 #pragma unroll
-                        for (int j = 0; j < 16; j++) {
-                            if constexpr (FLOAT_KEY)
-                                t[j] = acc[j];
-                            else
-                                t[j] = __mul24((int)acc[j], neg2) + nlr[tt][j];
+                            for (int j = 0; j < 16; j++) t[j] = acc[j];
+                        } else {
+                            const int4* cl = reinterpret_cast<const int4*>(a.b_nlc + (size_t)(ts + tt) * 32 + h * 16);
+#pragma unroll
+                            for (int c4 = 0; c4 < 4; c4++) {
+                                const int4 cv = cl[c4];
+                                t[4 * c4] = 2 * (int)acc[4 * c4] + cv.x;
+                                t[4 * c4 + 1] = 2 * (int)acc[4 * c4 + 1] + cv.y;
+                                t[4 * c4 + 2] = 2 * (int)acc[4 * c4 + 2] + cv.z;
+                                t[4 * c4 + 3] = 2 * (int)acc[4 * c4 + 3] + cv.w;
+                            }
                         }
                     }
                     const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
@@ -505,6 +516,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                                 i1[qb] = c1 ? row : i1[qb];
                             }
                             thr[qb] = min(thr[qb], m2[qb]);
+                            thr_h[qb] = half_up(thr[qb]);
                         }
                     }
                     // a real 2nd best found here bounds the query everywhere (not

@@ -71,7 +71,8 @@ inline size_t tile_bytes(int kind) { return (size_t)groups_per_tile(kind) * 64 *
 struct PackedSet {
     DevBuf tiles;  // [tiles][groups][64][16B]
     DevBuf norm;   // float[rows_pad] (int32 for KIND_I8), natural row order
-    DevBuf norm_lane;  // [tiles][32] norms in accumulator-lane order (base only)
+    DevBuf norm_lane;  // [tiles][32] norms in accumulator-lane order (base only; I8: see below)
+    DevBuf norm_half;  // KIND_I8 base, brute force: norm_lane >> 1 (the MFMA's initial value)
     DevBuf u8, n2;     // KIND_FP8: uint8 rows [rows][128] and int32 |x|^2 for the exact re-rank
     uint32_t rows = 0, rows_pad = 0, ntiles = 0;
     int kind = 0;
@@ -79,7 +80,10 @@ struct PackedSet {
 
 // Pack n rows of `type` input into `kind` tiles padded to rows_pad rows.
 // PACK_BASE: also writes norm_lane; pad rows get norm = +inf (INT32_MAX for
-//            I8) so they never rank.
+//            I8) so they never rank.  KIND_I8 stores base rows as b' = x - 128
+//            and queries as q'' = 127 - x = -(x - 128) - 1; the base's
+//            norm_lane holds c = |b'|^2 + 2 sum(b'), so the rank key
+//            |b'|^2 - 2 q'.b' equals 2 q''.b' + c.
 // PACK_QUERY: float kinds store -2q (the fused kernel's operand).
 // PACK_PLAIN: values as given (hyperplanes, projection inputs).
 // `bad` (device int) is OR-ed with 1 if a value is not an integer in [0,255].

@@ -238,7 +238,9 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         FusedArgs fa{};
         fa.q_tiles = static_cast<const uint4*>(qs.tiles.p);
         fa.b_tiles = static_cast<const uint4*>(bs.tiles.p);
-        fa.b_nl = static_cast<const float*>(bs.norm_lane.p);
+        // I8: the MFMAs start at c >> 1; the slow path adds c to twice the product
+        fa.b_nl = static_cast<const float*>(bs.kind == KIND_I8 ? bs.norm_half.p : bs.norm_lane.p);
+        fa.b_nlc = static_cast<const int*>(bs.norm_lane.p);
         fa.part = part + (size_t)p0[l] * q_stride;
         fa.bound = l > 0 ? bound : nullptr;
         fa.q_n = q_n;
@@ -249,7 +251,6 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         fa.tiles_per_slab = tps_slab[l];
         fa.n_slabs = slabs[l];
         fa.n_groups = n_groups;
-        fa.neg64 = -64;
         return fa;
     };
     auto mk_merge = [&](int n_part) {

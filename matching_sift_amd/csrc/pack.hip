@@ -101,7 +101,8 @@ template <int TYPE, int KIND>
 __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, uint32_t n,
                                                    uint32_t rows_pad, uint4* __restrict__ tiles,
                                                    float* __restrict__ norm,
-                                                   float* __restrict__ norm_lane, int role,
+                                                   float* __restrict__ norm_lane,
+                                                   int* __restrict__ norm_half, int role,
                                                    int* __restrict__ bad,
                                                    const int32_t* __restrict__ rowmap) {
     const bool is_base = role == PACK_BASE;
@@ -127,6 +128,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
     // scale); the norm below is computed from q itself.
     const float opscale = (role == PACK_QUERY && KIND != KIND_I8) ? -2.f : 1.f;
     const uint32_t t = row >> 5, i32 = row & 31;
+    int sum_b = 0;  // KIND_I8 base: sum of b' over the row
     constexpr int G = (KIND == KIND_F32) ? 17 : (KIND == KIND_I8 ? 4 : (KIND == KIND_FP8 ? 5 : 9));
     uint4* tile = tiles + (size_t)t * G * 64;
     float nrm;
@@ -218,19 +220,29 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
             tile[4 * 64 + i32 + 32 * blk] = make_uint4(sbyte, 0u, 0u, 0u);
         }
         nrm = s;
-    } else {  // KIND_I8: values shifted by -128; lane half h holds dims 32g + 16h + 0..15
-        int si = 0;
+    } else {  // KIND_I8: base b' = x - 128, queries q'' = 127 - x (both in int8); lane half h
+              // holds dims 32g + 16h + 0..15.  Base rows also sum b' (for c = |b'|^2 + 2 sum b').
+        int si = 0, sb = 0;
         uint32_t w[2] = {0u, 0u};
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            int x = (int)v[i] - 128;
+            const int x = (int)v[i] - 128;
             si += x * x;
-            w[i >> 2] |= ((uint32_t)(x & 0xFF)) << (8 * (i & 3));
+            sb += x;
+            const int op = role == PACK_QUERY ? -x - 1 : x;
+            w[i >> 2] |= ((uint32_t)(op & 0xFF)) << (8 * (i & 3));
         }
         const int g = blk >> 2, h = (blk >> 1) & 1, half = blk & 1;
         uint2* dst = reinterpret_cast<uint2*>(tile + g * 64 + i32 + 32 * h) + half;
         *dst = make_uint2(w[0], w[1]);
         nrm = __int_as_float(si);
+        if (is_base) {
+            sb += __shfl_xor(sb, 8, 16);
+            sb += __shfl_xor(sb, 4, 16);
+            sb += __shfl_xor(sb, 2, 16);
+            sb += __shfl_xor(sb, 1, 16);
+            sum_b = sb;
+        }
     }
     // reduce the 16 partial norms of this row (lanes 16k..16k+15)
     if constexpr (KIND == KIND_I8) {
@@ -254,8 +266,14 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
         norm[row] = nrm;
         if (is_base) {
             const int hl = (i32 >> 2) & 1, j = 4 * (i32 >> 3) + (i32 & 3);
-            const float pad = (KIND == KIND_I8) ? __int_as_float(PAD_KEY) : (float)PAD_KEY;
-            norm_lane[(size_t)t * 32 + hl * 16 + j] = real ? nrm : pad;
+            const size_t li = (size_t)t * 32 + hl * 16 + j;
+            if constexpr (KIND == KIND_I8) {
+                const int c = real ? __float_as_int(nrm) + 2 * sum_b : PAD_KEY;
+                norm_lane[li] = __int_as_float(c);
+                if (norm_half) norm_half[li] = c >> 1;
+            } else {
+                norm_lane[li] = real ? nrm : (float)PAD_KEY;
+            }
         }
     }
     // Norm group (float kinds): threads 0/1 of the row write lane halves 0/1.
@@ -356,21 +374,22 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
     auto* tiles = static_cast<uint4*>(o.tiles.p);
     auto* norm = static_cast<float*>(o.norm.p);
     auto* nl = static_cast<float*>(o.norm_lane.p);
+    auto* nh = static_cast<int*>(o.norm_half.p);
     switch (kind) {
         case KIND_BF16:
-            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
             break;
         case KIND_F16:
-            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
             break;
         case KIND_F32:
-            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
             break;
         case KIND_I8:
-            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
             break;
         case KIND_FP8:
-            pack_kernel<TYPE, KIND_FP8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_FP8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
             break;
         default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
     }
@@ -388,6 +407,7 @@ int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
     SIFT_TRY(reserve(h, o.tiles, (size_t)ntiles * tile_bytes(kind)));
     SIFT_TRY(reserve(h, o.norm, (size_t)rows_pad * 4));
     if (is_base) SIFT_TRY(reserve(h, o.norm_lane, (size_t)rows_pad * 4));
+    if (is_base && kind == KIND_I8) SIFT_TRY(reserve(h, o.norm_half, (size_t)rows_pad * 4));
     o.rows = n;
     o.rows_pad = rows_pad;
     o.ntiles = ntiles;
@@ -437,7 +457,7 @@ int pack_side(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
 }
 
 void release_set(PackedSet& s) {
-    for (DevBuf* b : {&s.tiles, &s.norm, &s.norm_lane, &s.u8, &s.n2}) release(*b);
+    for (DevBuf* b : {&s.tiles, &s.norm, &s.norm_lane, &s.norm_half, &s.u8, &s.n2}) release(*b);
 }
 
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,

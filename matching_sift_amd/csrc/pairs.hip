@@ -216,7 +216,7 @@ FusedArgs make_args(sift_pairs_plan* p, uint32_t* idx, float* dist, uint32_t* ma
     a.out_dist = dist;
     a.out_matches = matches;
     a.ratio = ratio;
-    a.neg64 = -64;
+    a.neg64 = 64;  // I8 composite = 64 q''.b' + (c << 5 | slot) = key * 32 + slot
     a.out_cand = static_cast<uint32_t*>(p->cand.p);
     return a;
 }
