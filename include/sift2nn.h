@@ -57,7 +57,7 @@ enum {
     SIFT_COMPUTE_BF16 = 1, /* BF16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F16 = 2,  /* FP16 MFMA, FP32 accumulate                     */
     SIFT_COMPUTE_F32 = 3,  /* FP32 MFMA                                      */
-    SIFT_COMPUTE_I8 = 4,   /* int8 MFMA on (x-128), int32 accumulate         */
+    SIFT_COMPUTE_I8 = 4,   /* int8 MFMA (x - 128, 127 - x), int32 accumulate */
     SIFT_COMPUTE_FP8 = 5   /* e4m3 coarse pass + exact re-rank, see below    */
 };
 /* SIFT_COMPUTE_FP8 (sift_knn2*, brute-force indexes, pairs plans): a coarse
