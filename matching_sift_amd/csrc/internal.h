@@ -143,6 +143,10 @@ bool device_accessible(const void* p);
 // of table `table` (combine: prefix the table index above the b code bits).
 int mlp_forward(sift_mlp* m, const PackedSet& xs, uint32_t n, float* h_out, float* z_out, uint32_t* codes,
                 uint32_t stride, int table, bool combine);
+// The same forward for L networks of equal width in one launch per 16: codes
+// (and z, [table][stride][32], if non-null) of every table.
+int mlp_forward_tables(sift_mlp* const* ms, int L, const PackedSet& xs, uint32_t n, float* z_out,
+                       uint32_t* codes, uint32_t stride, bool combine);
 int mlp_nbits(const sift_mlp* m);
 
 }  // namespace sift

@@ -452,10 +452,7 @@ int hash_codes(sift_context* h, const sift_lsh_index* ix, const PackedSet& xs, u
                uint32_t stride, bool combine, float* margins = nullptr) {
     if (ix->hash_kind != SIFT_LSH_LEARNED)
         return project(h, xs, n, ix->hp, ix->L, ix->nbits, codes, stride, combine, margins);
-    for (int t = 0; t < ix->L; t++)
-        SIFT_TRY(mlp_forward(ix->mlps[t], xs, n, nullptr, margins ? margins + (size_t)t * stride * 32 : nullptr, codes,
-                             stride, t, combine));
-    return SIFT_OK;
+    return mlp_forward_tables(ix->mlps.data(), ix->L, xs, n, margins, codes, stride, combine);
 }
 
 // Multi-probe codes: per (table, query) the base code, then the code with

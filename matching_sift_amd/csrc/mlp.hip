@@ -71,7 +71,7 @@ struct FwdArgs {
     int table, combine;
 };
 
-__global__ __launch_bounds__(256) void mlp_fwd(FwdArgs a) {
+__device__ __forceinline__ void mlp_fwd_body(const FwdArgs& a) {
     const int lane = threadIdx.x & 63, h = lane >> 5;
     const uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (tile >= a.n_tiles) return;  // whole wave
@@ -125,6 +125,35 @@ __global__ __launch_bounds__(256) void mlp_fwd(FwdArgs a) {
             a.codes[(size_t)a.table * a.stride + row] =
                 (a.combine && a.nbits < 32) ? (((uint32_t)a.table << a.nbits) | code) : code;
     }
+}
+
+__global__ __launch_bounds__(256) void mlp_fwd(FwdArgs a) { mlp_fwd_body(a); }
+
+// Several networks in one launch (one per grid row y): the learned index's
+// tables hash a small query batch with one launch instead of one each.
+constexpr int FWD_NETS = 16;
+struct NetPtrs {
+    const uint4* w1p;
+    const float* b1;
+    const uint4* w2p;
+    const float* b2;
+};
+struct FwdMulti {
+    FwdArgs a;  // common fields; w1p/b1/w2p/b2/table/z_out taken per network
+    NetPtrs nets[FWD_NETS];
+    int t0;  // table of grid row 0
+};
+
+__global__ __launch_bounds__(256) void mlp_fwd_multi(FwdMulti m) {
+    FwdArgs a = m.a;
+    const int y = blockIdx.y;
+    a.w1p = m.nets[y].w1p;
+    a.b1 = m.nets[y].b1;
+    a.w2p = m.nets[y].w2p;
+    a.b2 = m.nets[y].b2;
+    a.table = m.t0 + y;
+    if (a.z_out) a.z_out += (size_t)a.table * a.stride * BMAX;
+    mlp_fwd_body(a);
 }
 
 // W1 [HID][128] -> BF16 A operands: element (u, d) at tile u/32, group d/16,
@@ -368,6 +397,38 @@ int mlp_forward(sift_mlp* m, const PackedSet& xs, uint32_t n, float* h_out, floa
     a.combine = combine ? 1 : 0;
     mlp_fwd<<<(a.n_tiles + 3) / 4, 256, 0, h->stream>>>(a);
     SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+// Codes (and z, [table][stride][BMAX]) of n rows under L networks of equal
+// width, FWD_NETS networks per launch.
+int mlp_forward_tables(sift_mlp* const* ms, int L, const PackedSet& xs, uint32_t n, float* z_out,
+                       uint32_t* codes, uint32_t stride, bool combine) {
+    if (n == 0 || L == 0) return SIFT_OK;
+    sift_context* h = ms[0]->h;
+    FwdMulti fm{};
+    fm.a.x_tiles = static_cast<const uint4*>(xs.tiles.p);
+    fm.a.tg = groups_per_tile(xs.kind);
+    fm.a.n_rows = n;
+    fm.a.n_tiles = (n + 31) / 32;
+    fm.a.nbits = ms[0]->nbits;
+    fm.a.h_out = nullptr;
+    fm.a.z_out = z_out;
+    fm.a.codes = codes;
+    fm.a.stride = stride;
+    fm.a.combine = combine ? 1 : 0;
+    for (int t0 = 0; t0 < L; t0 += FWD_NETS) {
+        const int nt = std::min(FWD_NETS, L - t0);
+        for (int k = 0; k < nt; k++) {
+            const sift_mlp* m = ms[t0 + k];
+            if (m->nbits != fm.a.nbits) return set_error(h, SIFT_ERR_INVALID, "networks differ in bits");
+            fm.nets[k] = NetPtrs{static_cast<const uint4*>(m->w1p.p), static_cast<const float*>(m->params.p) + OFF_B1,
+                                 static_cast<const uint4*>(m->w2p.p), static_cast<const float*>(m->params.p) + OFF_B2};
+        }
+        fm.t0 = t0;
+        mlp_fwd_multi<<<dim3((fm.a.n_tiles + 3) / 4, (unsigned)nt), 256, 0, h->stream>>>(fm);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     return SIFT_OK;
 }
 
