@@ -132,14 +132,37 @@ __global__ void iota_mod(uint32_t* __restrict__ v, uint32_t total, uint32_t n) {
 // (accept_ratio > 0) a best candidate that does not pass the ratio test
 // sqrt(d1)/sqrt(d2) < accept_ratio against the next one (IEEE sqrt and
 // division, as the match output).
-__global__ void select_low(const uint32_t* __restrict__ nc, const float* __restrict__ d, uint32_t q_n,
-                           uint32_t min_cand, float accept_ratio, uint32_t* __restrict__ count,
-                           uint32_t* __restrict__ list) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= q_n) return;
+// Queries the LSH answer does not settle: fewer than min_cand candidates, or
+// (accept_ratio > 0) a best candidate failing the ratio test against the next.
+__device__ inline bool needs_exact(const uint32_t* nc, const float* d, uint32_t q, uint32_t min_cand,
+                                   float accept_ratio) {
     bool low = nc[q] < min_cand;
     if (accept_ratio > 0.f) low |= !(sqrtf(d[2 * (size_t)q]) / sqrtf(d[2 * (size_t)q + 1]) < accept_ratio);
-    if (low) list[atomicAdd(count, 1u)] = q;
+    return low;
+}
+
+__global__ void fallback_flags(const uint32_t* __restrict__ nc, const float* __restrict__ d, uint32_t q_n,
+                               uint32_t min_cand, float accept_ratio, uint32_t* __restrict__ flag) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < q_n) flag[q] = needs_exact(nc, d, q, min_cand, accept_ratio) ? 1u : 0u;
+}
+
+// pos = exclusive scan of the flags: the fallback queries in increasing order
+// become map[0 .. count); the last thread writes count.
+__global__ void fallback_compact(const uint32_t* __restrict__ nc, const float* __restrict__ d, uint32_t q_n,
+                                 uint32_t min_cand, float accept_ratio, const uint32_t* __restrict__ pos,
+                                 int32_t* __restrict__ map, uint32_t* __restrict__ count) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= q_n) return;
+    const bool low = needs_exact(nc, d, q, min_cand, accept_ratio);
+    if (low) map[pos[q]] = (int32_t)q;
+    if (q == q_n - 1) *count = pos[q] + (low ? 1u : 0u);
+}
+
+// Pad rows of the fallback query pack: map[cnt .. rows_pad) = -1.
+__global__ void map_tail(int32_t* __restrict__ map, uint32_t cnt, uint32_t rows_pad) {
+    const uint32_t i = cnt + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < rows_pad) map[i] = -1;
 }
 
 // Exact results of the fallback queries into the caller's outputs.
@@ -952,28 +975,30 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
         dq = static_cast<float*>(sc->fb_dist.p);
     }
     SIFT_TRY(sift_lsh_query(h, ix, q, type, q_n, idx, dq, nc, matches, ratio, 0));
-    // queries the LSH answer does not settle
+    // queries the LSH answer does not settle, in increasing order (flags,
+    // scan, compaction on the device; only the count comes back)
     SIFT_TRY(reserve(h, sc->fb_list, (size_t)q_n * 4 + 4));
-    auto* list = static_cast<uint32_t*>(sc->fb_list.p);
+    auto* pos = static_cast<uint32_t*>(sc->fb_list.p);
     SIFT_TRY(reserve(h, sc->fb_count, 4));
-    SIFT_HIP_TRY(h, hipMemsetAsync(sc->fb_count.p, 0, 4, h->stream));
-    select_low<<<(q_n + 255) / 256, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio,
-                                                         static_cast<uint32_t*>(sc->fb_count.p), list);
+    SIFT_TRY(reserve(h, sc->fb_map, (size_t)query_rows_pad(bf->kind, q_n) * 4));
+    auto* map = static_cast<int32_t*>(sc->fb_map.p);
+    const unsigned gq = (q_n + 255) / 256;
+    fallback_flags<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos);
+    SIFT_HIP_TRY(h, hipGetLastError());
+    SIFT_TRY(exclusive_scan(h, pos, q_n, sc->ulevels, 0));
+    fallback_compact<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos, map,
+                                                static_cast<uint32_t*>(sc->fb_count.p));
     SIFT_HIP_TRY(h, hipGetLastError());
     uint32_t cnt = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     if (n_fallback) *n_fallback = cnt;
     if (cnt == 0) return SIFT_OK;
-    // the list comes back in arbitrary order: sort it so the rerun is deterministic
-    std::vector<uint32_t> hl(cnt);
-    SIFT_HIP_TRY(h, hipMemcpy(hl.data(), list, (size_t)cnt * 4, hipMemcpyDeviceToHost));
-    std::sort(hl.begin(), hl.end());
     const uint32_t rows_pad = query_rows_pad(bf->kind, cnt);
-    std::vector<int32_t> map(rows_pad, -1);
-    for (uint32_t i = 0; i < cnt; i++) map[i] = (int32_t)hl[i];
-    SIFT_TRY(reserve(h, sc->fb_map, (size_t)rows_pad * 4));
-    SIFT_HIP_TRY(h, hipMemcpy(sc->fb_map.p, map.data(), (size_t)rows_pad * 4, hipMemcpyHostToDevice));
+    if (rows_pad > cnt) {
+        map_tail<<<(rows_pad - cnt + 255) / 256, 256, 0, h->stream>>>(map, cnt, rows_pad);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     // exact 2-NN of those queries against the packed base, then scatter back
     PackedSet& qs = sc->fb_q;
     SIFT_TRY(reserve(h, qs.tiles, (size_t)(rows_pad / 32) * tile_bytes(bf->kind)));
