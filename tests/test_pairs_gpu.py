@@ -157,6 +157,39 @@ def test_pairs_config3_sample(handle):
     plan.close()
 
 
+@pytest.mark.parametrize("seed", [5, 9, 13])
+def test_pairs_random_batches(handle, seed):
+    """Random batches: 1-5 images a side with sizes at tile and group edges
+    (empty images included), 1-5 pairs that may repeat, natural or
+    3-letter-alphabet data; every exact kind, one-shot."""
+    rng = np.random.default_rng(seed)
+    q_pick = [0, 1, 5, 31, 32, 33, 100, 700, 2049]
+    r_pick = [0, 1, 2, 31, 32, 33, 100, 900, 3000]
+    done = 0
+    while done < 12:
+        qs = [int(rng.choice(q_pick)) for _ in range(int(rng.integers(1, 6)))]
+        rs = [int(rng.choice(r_pick)) for _ in range(int(rng.integers(1, 6)))]
+        n_p = int(rng.integers(1, 6))
+        pairs = np.stack([rng.integers(0, len(qs), n_p), rng.integers(0, len(rs), n_p)], 1).astype(np.uint32)
+        if sum(qs) == 0 or sum(rs) == 0 or sum(qs[a] for a, _ in pairs) == 0:
+            continue
+        ties = bool(rng.integers(0, 2))
+        if ties:
+            q = (rng.integers(0, 3, size=(sum(qs), 128)) * 85).astype(np.float32)
+            r = (rng.integers(0, 3, size=(sum(rs), 128)) * 85).astype(np.float32)
+        else:
+            q = datagen.base(sum(qs), seed=int(rng.integers(1 << 30)))
+            r = datagen.base(sum(rs), seed=int(rng.integers(1 << 30)))
+        q_off = np.concatenate([[0], np.cumsum(qs)]).astype(np.uint32)
+        r_off = np.concatenate([[0], np.cumsum(rs)]).astype(np.uint32)
+        exp = expected_pairs(q, q_off, r, r_off, pairs)
+        for c in KINDS:
+            say("random batch", seed, done, "ties" if ties else "natural", qs, rs, pairs.tolist(), "kind", c)
+            assert_same(match_pairs(handle, q, q_off, r, r_off, pairs, compute=c), exp,
+                        "batch %d kind %d" % (done, c))
+        done += 1
+
+
 def test_match_pairs_oneshot(handle):
     q, q_off, r, r_off, pairs = ragged_batch(6)
     exp = expected_pairs(q, q_off, r, r_off, pairs)
