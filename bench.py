@@ -13,7 +13,7 @@ Also reported on the same line:
                  duration of the step's fused_top2 launches (a seeded chain
                  over consecutive base ranges), HIP events on the library's
                  stream
-  cpu_baseline : the oracle's host_brute restatement (oracle/) on the box's
+  cpu_baseline : the oracle's host_brute restatement (oracle/) on this machine's
                  host cores over a bounded query sample against the full base
   index_qps    : queries/sec with the base packed once (sift_bf_index_*)
   host_input_qps: queries/sec with both sets in page-locked host memory
