@@ -221,3 +221,41 @@ def test_lsh_bucket_stat(handle, nbits, L):
                for t in range(L)) / (L * len(b))
     assert ix.bucket_stat() == pytest.approx(want, rel=1e-12)
     ix.close()
+
+
+@pytest.mark.parametrize("seed", [3, 17])
+def test_lsh_random_configs(handle, monkeypatch, seed):
+    """Random (nbits, L, probes, hyperplane kind) with base and query sizes at
+    wave and tile edges, natural or 3-letter-alphabet data (many equal
+    projections and distances): codes equal the oracle's, and both re-rank
+    schedules return the oracle's LSH query over the index's probe codes."""
+    rng = np.random.default_rng(seed)
+    for it in range(16):
+        nbits, L = int(rng.integers(1, 33)), int(rng.integers(1, 7))
+        while nbits + int(np.ceil(np.log2(L))) > 32:  # table number shares the 32-bit key
+            L -= 1
+        P = int(rng.integers(0, min(nbits, 16) + 1)) if rng.random() < 0.5 else 0
+        n = int(rng.choice([1, 2, 31, 32, 33, 700, 5000]))
+        qn = int(rng.choice([1, 7, 64, 65, 300]))
+        ties = bool(rng.integers(0, 2))
+        hk = LSH_HYPERPLANE_PM1 if rng.random() < 0.3 else LSH_HYPERPLANE
+        if ties:
+            b = (rng.integers(0, 3, size=(n, 128)) * 85).astype(np.float32)
+            q = (rng.integers(0, 3, size=(qn, 128)) * 85).astype(np.float32)
+        else:
+            b = datagen.base(n, seed=int(rng.integers(1 << 30)))
+            q = datagen.queries(b, qn, seed=int(rng.integers(1 << 30)))
+        say("lsh random", seed, it, "nbits", nbits, "L", L, "P", P, "n", n, "q", qn, "ties", ties, "kind", hk)
+        ix = LSHIndex(handle, b, nbits, L, hash_kind=hk, seed=int(rng.integers(1 << 20)))
+        ix.set_probes(P)
+        cb = ix.codes(b)
+        np.testing.assert_array_equal(cb, oracle_tables(b, ix.hyperplanes()))
+        pc = ix.probe_codes(q) if P else ix.codes(q)[:, None]
+        cq = pc.reshape(L * (P + 1), -1)
+        oi, od, _ = oracle.lsh_query(q, b, cq, np.repeat(cb, P + 1, axis=0))
+        for order in ("0", "1"):
+            monkeypatch.setenv("SIFT_LSH_ORDER", order)
+            idx, dist, _, _ = ix.query(q)
+            np.testing.assert_array_equal(idx, oi, err_msg="order %s" % order)
+            np.testing.assert_array_equal(dist, np.where(oi == 0xFFFFFFFF, FMAX, od.astype(np.float32)))
+        ix.close()
