@@ -22,6 +22,16 @@
  *   - Distances are squared L2.  For descriptors holding integers in [0,255]
  *     every compute kind except FP8 returns bit-exact indices and distances
  *     (ties go to the lowest base index).
+ *   - Other data (e.g. unit-norm floats, SIFT / 512) with the float kinds
+ *     (BF16, F16, F32): the MFMA pass ranks coarsely (values rounded to the
+ *     kind), the 16 best coarse candidates of each query are re-ranked with
+ *     the reference host_lenght's sequential FP32 sum of (q_i - r_i)^2 over
+ *     the rows as given, and those are the results.  Distances are then
+ *     bit-equal to host_brute's for the returned rows, and the indices are
+ *     host_brute's unless a true 2-NN row is not among the candidates: this
+ *     needs another row of the same slab (or 16 others) within the coarse
+ *     rounding of its distance (2^-6 (|q| + |r|)^2 for BF16, 2^-9 for F16,
+ *     2^-16 for F32).  I8 and FP8 return SIFT_ERR_NOT_EXACT on such data.
  */
 #ifndef SIFT2NN_H
 #define SIFT2NN_H
@@ -34,6 +44,9 @@ extern "C" {
 #endif
 
 typedef struct sift_context* sift_handle_t;
+
+/* One descriptor: 128 floats (include/knn_brute.h:5 of the reference). */
+typedef float des_t_f[128];
 
 /* Layout-identical to CUDA/HIP float4 as used by the reference's sorted[]
  * output: x = d1, y = d2, z = (float)idx1, w = (float)idx2. */
@@ -129,19 +142,20 @@ int cublas_2nn_sift_batch(void* q_points, void* Q, int type, void* R, uint32_t q
                           void* stream, int warps);
 
 /* Replaces void device_brute(...) (include/knn_brute.h:14, src/knn_brute.cu:619).
- * Device float inputs, device float4 output {d1, d2, idx1, idx2}; no handle
- * (a per-device default handle is used).  Result: sift_last_status(). */
-void device_brute(const float* q_points, const float* r_points, int q_n, int r_n,
-                  sift_float4* sorted);
+ * Device float inputs, device float4 output {d1, d2, idx1, idx2} (sorted: a
+ * float4 or sift_float4 array); no handle (a per-device default handle is
+ * used).  Exact FP32 semantics of host_brute (see the conventions above).
+ * Result: sift_last_status(). */
+void device_brute(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted);
 
 /* Replaces void cublas_2nn_f(...) (include/knn_brute.h:18, src/knn_brute.cu:523). */
-void cublas_2nn_f(const float* q_points, const float* r_points, int q_n, int r_n,
-                  sift_float4* sorted, sift_handle_t handle);
+void cublas_2nn_f(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted,
+                  sift_handle_t handle);
 
 /* Replaces void cublas_2nn_brute_f(...) (include/knn_brute.h:17, src/knn_brute.cu:567).
  * dist is the reference's caller-owned distance buffer; unused, may be NULL. */
-void cublas_2nn_brute_f(const float* q_points, const float* r_points, int q_n, int r_n,
-                        sift_float4* sorted, float* dist, sift_handle_t handle);
+void cublas_2nn_brute_f(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted,
+                        float* dist, sift_handle_t handle);
 
 /* Replaces int lsh_gpu(...) (include/lsh_h.h:69, src/lsh_h.cu:722): build an
  * LSH index over r with l tables of nbits bits, query it with q, write
@@ -163,8 +177,12 @@ int sift_set_seed(sift_handle_t handle, uint64_t seed);
  *   dist float[q_n][2]   squared L2 (FLT_MAX when absent)
  *   sorted sift_float4[q_n]  reference float4 layout
  *   matches uint32[q_n]  idx1 if sqrt(d1)/sqrt(d2) < ratio else UINT32_MAX
- * compute: SIFT_COMPUTE_*.  BF16/F16/I8 require integer data in [0,255]
- * (checked; SIFT_ERR_NOT_EXACT otherwise). */
+ * compute: SIFT_COMPUTE_*.  I8 and FP8 require integer data in [0,255]
+ * (checked: SIFT_ERR_NOT_EXACT otherwise); BF16, F16 and F32 take any data
+ * (exact re-rank of coarse candidates, see the conventions above).  The
+ * async variant returns without checking: errors of its launches (input
+ * outside an integer kind's domain, a failed wait) are reported by the next
+ * blocking call on the handle or by sift_synchronize. */
 int sift_knn2(sift_handle_t handle, const void* q, const void* r, int type, uint32_t q_n,
               uint32_t r_n, int compute, uint32_t* idx, float* dist, sift_float4* sorted,
               uint32_t* matches, float ratio);
@@ -173,7 +191,10 @@ int sift_knn2_async(sift_handle_t handle, const void* q, const void* r, int type
                     uint32_t* matches, float ratio);
 
 /* Brute-force index: the base packed once in the MFMA operand layout, queried
- * many times (removes the per-call base conversion of the reference). */
+ * many times (removes the per-call base conversion of the reference).  Only
+ * F32 indexes take a base that is not all integers in [0,255] (the others
+ * return SIFT_ERR_NOT_EXACT at build: the packed rows are all they keep);
+ * queries may be any data for BF16, F16 and F32 indexes. */
 typedef struct sift_bf_index* sift_bf_index_t;
 int sift_bf_index_build(sift_handle_t handle, const void* r, int type, uint32_t r_n, int compute,
                         sift_bf_index_t* index);
@@ -284,6 +305,12 @@ int sift_lsh_probe_codes(sift_handle_t h, sift_lsh_index_t index, const void* q_
  * copies of its networks.  No counterpart in the reference. */
 int sift_lsh_save(sift_lsh_index_t index, const char* path);
 int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* index);
+/* Check an index file on the host without loading it (sift_lsh_load runs the
+ * same checks): header, size, row ids below the base size, every table's keys
+ * sorted under its own prefix, stored norms equal to their rows, hyperplane
+ * values integers in [-127, 127].  SIFT_ERR_INVALID with the reason in
+ * sift_last_error(NULL) otherwise. */
+int sift_lsh_check_file(const char* path);
 /* LSH index whose table t hashes with network mlps[t] (all of the same b);
  * the networks must outlive the index.  Query it with sift_lsh_query. */
 int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_n,

@@ -13,7 +13,8 @@ from ._lib import (  # noqa: F401
     LSH_HYPERPLANE, LSH_LEARNED, LSH_HYPERPLANE_PM1,
 )
 from .api import (  # noqa: F401
-    DeviceArray, Handle, to_device, knn2, cublas_2nn_sift, device_brute, cublas_2nn_f,
+    DeviceArray, PinnedArray, Handle, to_device, knn2, cublas_2nn_sift, cublas_2nn_sift_batch,
+    device_brute, cublas_2nn_f, cublas_2nn_brute_f, lsh_check_file,
     BruteForceIndex, LSHIndex, lsh_gpu, PairsPlan, match_pairs, PAIRS_GRAPH, PAIRS_AUTOTUNE,
     HashMLP, TrainOpts, MLP_NPARAM, vecs_info, read_vecs, recall_count,
 )

@@ -77,6 +77,7 @@ SIGNATURES = {
     "sift_lsh_probe_codes": (i32, [vp, vp, vp, i32, u32, vp]),
     "sift_lsh_save": (i32, [vp, C.c_char_p]),
     "sift_lsh_load": (i32, [vp, C.c_char_p, C.POINTER(vp)]),
+    "sift_lsh_check_file": (i32, [C.c_char_p]),
     "sift_lsh_query_fallback": (i32, [vp, vp, vp, vp, i32, u32, u32, f32, vp, vp, vp, vp, f32, vp]),
     "sift_pairs_plan_create": (i32, [vp, i32, i32, u32, vp, u32, vp, u32, vp, C.c_uint, C.POINTER(vp)]),
     "sift_pairs_plan_execute": (i32, [vp, vp, vp, vp, vp, vp, vp, f32, i32]),

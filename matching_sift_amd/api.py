@@ -62,6 +62,44 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """Page-locked, device-mapped host memory (sift_host_alloc), viewed as a
+    numpy array: the FLOAT_HOST input of the reference's caller
+    (src/test.cu:63-64 allocates its descriptors this way)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().sift_host_alloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+        buf = (C.c_char * max(1, self.nbytes)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=self.dtype, count=int(np.prod(self.shape))).reshape(self.shape)
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        out = cls(arr.shape, arr.dtype)
+        out.array[...] = arr
+        return out
+
+    def numpy(self):
+        return self.array.copy()
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            lib().sift_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def to_device(arr):
     return arr if isinstance(arr, DeviceArray) else DeviceArray.from_numpy(arr)
 
@@ -465,6 +503,25 @@ def cublas_2nn_sift(q_points, r_points, type_, q_n, r_n, matches, threshold, han
     """src/knn_brute.cu:24 — matches[i] = idx1 if sqrt(d1)/sqrt(d2) < threshold."""
     return check(lib().cublas_2nn_sift(q_points.ptr, r_points.ptr, type_, q_n, r_n, matches.ptr,
                                        float(threshold), handle.h, stream_n, use, warps), handle.h)
+
+
+def cublas_2nn_sift_batch(q_points, type_, R, q_n, r_n, matches, threshold, handle, stream=None, warps=1):
+    """src/knn_brute.cu:213 — q typed by type_ (float or half, device), R fp16
+    on the device; Q and dist (the reference's caller scratch) are not needed."""
+    return check(lib().cublas_2nn_sift_batch(q_points.ptr, None, type_, R.ptr, q_n, r_n, None, matches.ptr,
+                                             float(threshold), handle.h, stream, warps), handle.h)
+
+
+def cublas_2nn_brute_f(q_points, r_points, q_n, r_n, sorted_, handle, dist=None):
+    """src/knn_brute.cu:567 — as cublas_2nn_f; the caller's dist buffer is unused."""
+    lib().cublas_2nn_brute_f(q_points.ptr, r_points.ptr, q_n, r_n, sorted_.ptr,
+                             dist.ptr if dist is not None else None, handle.h)
+    return check(lib().sift_last_status(), handle.h)
+
+
+def lsh_check_file(path):
+    """Check an LSH index file on the host (no device needed); raises SiftError."""
+    return check(lib().sift_lsh_check_file(str(path).encode()))
 
 
 def device_brute(q_points, r_points, q_n, r_n, sorted_):

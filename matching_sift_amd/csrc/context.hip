@@ -1,4 +1,5 @@
 // context.hip — handle lifetime, errors, device-memory plumbing, profiling.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -45,6 +46,24 @@ void release(DevBuf& b) {
     if (b.p) hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+int* wait_status(sift_context* h) { return static_cast<int*>(h->flag.p) + 2; }
+
+int wait_error(sift_context* h, int status, const char* what) {
+    if (!status) return SIFT_OK;
+    SIFT_HIP_TRY(h, hipMemsetAsync(wait_status(h), 0, 4, h->stream));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return set_error(h, SIFT_ERR_HIP,
+                     std::string(what) + ": a stage hand-off wait of the fused kernel gave up after " +
+                         std::to_string(h->wait_polls) + " polls; the results of that launch are not valid");
+}
+
+int check_wait(sift_context* h, const char* what) {
+    int st = 0;
+    SIFT_HIP_TRY(h, hipMemcpyAsync(&st, wait_status(h), 4, hipMemcpyDeviceToHost, h->stream));
+    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    return wait_error(h, st, what);
 }
 
 const char* prof_name(int k) {
@@ -136,6 +155,15 @@ int sift_create(sift_handle_t* handle) {
         hipEventCreate(&h->pend[i].a);
         hipEventCreate(&h->pend[i].b);
     }
+    // flags and the sticky wait status exist from the start, zeroed
+    if (reserve(h, h->flag, 16) != SIFT_OK || hipMemset(h->flag.p, 0, 16) != hipSuccess) {
+        sift_destroy(h);
+        return set_error(nullptr, SIFT_ERR_HIP, "sift_create: status buffer");
+    }
+    if (const char* env = std::getenv("SIFT_WAIT_POLLS")) {
+        const long v = std::strtol(env, nullptr, 10);
+        h->wait_polls = (int)std::max(0L, std::min(v, 1L << 24));
+    }
     *handle = h;
     tl_status = SIFT_OK;
     return SIFT_OK;
@@ -166,7 +194,7 @@ int sift_last_status(void) { return tl_status; }
 int sift_synchronize(sift_handle_t h) {
     if (!h) return set_error(nullptr, SIFT_ERR_INVALID, "null handle");
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-    return SIFT_OK;
+    return check_wait(h, "sift_synchronize");
 }
 
 void* sift_get_stream(sift_handle_t h) { return h ? (void*)h->stream : nullptr; }

@@ -183,6 +183,14 @@ struct FusedArgs {
     float ratio;
     int neg64;  // pairs kernel, I8: 64, passed at run time so a composite takes one v_mad_i32_i24
     uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
+    // 8-wave blocks: polls allowed per stage hand-off wait, and the status word
+    // a wave ORs 1 into when a wait runs out of them (its results are then void)
+    int wait_polls;
+    int* status;
+    // brute force, float kinds: device flag, nonzero when some input value is
+    // not an integer in [0,255]; the keys are then coarse and the shared bound
+    // is not lowered inside the launch (null: exact keys)
+    const int* coarse;
 };
 
 

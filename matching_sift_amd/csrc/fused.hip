@@ -131,13 +131,16 @@ __device__ inline void barrier_only() {
 }
 
 // Stage hand-off counters in LDS (see fused_slab): wait until *c reaches
-// target, sleeping between polls.  The poll count is capped (about 2^16 x 64
-// cycles), so a wave can never wait forever.
-__device__ inline void wait_count(const uint32_t* c, uint32_t target) {
-    for (int it = 0; it < (1 << 16); it++) {
-        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target) break;
+// target, sleeping between polls.  The poll count is capped (`polls`, about 64
+// cycles each), so a wave can never wait forever; a wave that gives up ORs 1
+// into *status, and the host reports the launch as failed.
+__device__ inline void wait_count(const uint32_t* c, uint32_t target, int polls, int* status) {
+    bool done = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target;
+    for (int it = 0; !done && it < polls; it++) {
         __builtin_amdgcn_s_sleep(1);
+        done = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= target;
     }
+    if (!done && (threadIdx.x & 63) == 0) atomicOr(status, 1);
     asm volatile("" ::: "memory");
 }
 // One increment per wave (lane 0), after everything before it in program order.
@@ -196,6 +199,10 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     const int lane = tid & 63, h = lane >> 5;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar address arithmetic
     const int qblk0 = task.qblk0 + wave * QW;  // this wave's first query block
+    // A lane's new 2nd best lowers the shared bound, except for coarse keys
+    // (FP8, or float kinds on data that is not all integers in [0,255]): an
+    // exact re-rank decides there, so only the widened seed bounds apply.
+    const bool tighten = KIND != KIND_FP8 && a.bound != nullptr && !(a.coarse != nullptr && *a.coarse != 0);
 
     uint4 qop[QW][G];
 #pragma unroll
@@ -300,7 +307,7 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         const int ts = t_begin + k * TPS;
         const int ntile = min(TPS, t_end - ts);
         const int buf = k % 3;
-        if constexpr (HANDOFF) wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)));
+        if constexpr (HANDOFF) wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)), a.wait_polls, a.status);
         uint4 afr[TPS][G];
         int nlr[TPS][16];
         // (tiles past the slab end read stale LDS: in bounds, never flagged)
@@ -327,7 +334,8 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         const int b2 = (k + 2) % 3;
         const bool issued = k + 2 < nst;
         if constexpr (HANDOFF) {
-            if (issued && k >= 1) wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)));
+            if (issued && k >= 1)
+                wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)), a.wait_polls, a.status);
         }
         const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
         if constexpr (HANDOFF) {
@@ -520,10 +528,9 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                         }
                     }
                     // a real 2nd best found here bounds the query everywhere (not
-                    // for FP8: its coarse keys may rank the true 2-NN above a
-                    // coarse 2nd best, so only the widened seed bounds apply,
-                    // and the candidates do not depend on block timing)
-                    if (KIND != KIND_FP8 && a.bound && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
+                    // for coarse keys: they may rank the true 2-NN above a coarse
+                    // 2nd best, and the candidates must not depend on block timing)
+                    if (tighten && i2[qb] != old2 && i2[qb] != 0xFFFFFFFFu) {
                         const uint32_t q = (uint32_t)(qblk0 + qb) * 32u + lq;
                         if (q - task.q_row0 < task.q_count) atomicMin(&a.bound[q], Key<KIND>::ord(m2[qb]));
                     }

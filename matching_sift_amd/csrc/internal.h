@@ -35,6 +35,13 @@ struct DevBuf {
 };
 int reserve(sift_context* h, DevBuf& b, size_t bytes);
 void release(DevBuf& b);
+// Device status word of the handle (h->flag word 2): a fused-kernel wave whose
+// stage hand-off wait ran out of polls ORs 1 into it.  check_wait syncs the
+// handle's stream, reads it, clears it and returns SIFT_ERR_HIP if it was set.
+int* wait_status(sift_context* h);
+int check_wait(sift_context* h, const char* what);
+// The same check on a status value already copied to the host.
+int wait_error(sift_context* h, int status, const char* what);
 
 // ---------------------------------------------------------- profiling ----
 enum ProfKernel { PROF_PACK = 0, PROF_FUSED, PROF_MERGE, PROF_SEED, PROF_LSH_PROJ, PROF_LSH_SORT,
@@ -113,16 +120,52 @@ int fp8_rerank_launch(sift_context* h, const uint32_t* cand, const void* q_u8, c
 // Check that every value is an integer in [0,255]; result in *bad_dev (OR-ed).
 int check_exact(sift_context* h, const void* in, int type, uint32_t n, int* bad_dev);
 
+// Rows for the exact FP32 re-rank of non-integer data: the caller's input
+// rows (types 1..4), or for the base a packed set's tiles: F32 (any data),
+// BF16 / F16 (integer data in [0,255], held exactly).
+constexpr int SRC_F32_TILES = 100;
+constexpr int SRC_BF16_TILES = 101;
+constexpr int SRC_F16_TILES = 102;
+struct ExactSrc {
+    const void* q = nullptr;
+    int q_type = 0;
+    const void* r = nullptr;
+    int r_type = 0;
+    const int32_t* q_map = nullptr;  // query row of output i: q_map[i] (null: i)
+};
 struct Knn2Out {
     uint32_t* idx = nullptr;     // [q][2]
     float* dist = nullptr;       // [q][2]
     sift_float4* sorted = nullptr;
     uint32_t* matches = nullptr;
     float ratio = 0.8f;
+    // Float kinds: device flag, nonzero when some input value is not an
+    // integer in [0,255] (the pack kernels set it).  The fused pass then only
+    // ranks coarsely: seed bounds are widened by the kind's rounding error,
+    // the 16 best coarse candidates of each query are re-ranked with the
+    // sequential FP32 sum of host_brute over `src`, and those results are the
+    // outputs.  Null: integer data assumed (exact keys).
+    const int* coarse = nullptr;
+    ExactSrc src;
 };
-// Fused distance + top-2 over packed sets, then merge into the outputs.
+// Caller-provided device scratch for knn2_packed: the slab partials and the
+// per-query bounds.  With it knn2_packed allocates nothing and never times
+// launch shapes (it uses the shape already picked on the handle, or the
+// kind's default).
+struct Knn2Work {
+    void* part = nullptr;
+    size_t part_bytes = 0;
+    int* bound = nullptr;
+    size_t bound_bytes = 0;
+};
+// Fused distance + top-2 over packed sets, then merge into the outputs
+// (work: null = the handle's grow-only buffers).
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
-                uint32_t r_n, const Knn2Out& out);
+                uint32_t r_n, const Knn2Out& out, const Knn2Work* work = nullptr);
+// Scratch bytes knn2_packed with a Knn2Work needs: partials (returned) and
+// bounds (*bound_bytes), for the shape it would use on this handle.
+size_t knn2_work_bytes(sift_context* h, int kind, uint32_t n_tiles, uint32_t q_n, uint32_t r_n,
+                       size_t* bound_bytes);
 // Packed query rows for q_n queries (whole launch groups).
 uint32_t query_rows_pad(int kind, uint32_t q_n);
 
@@ -157,7 +200,14 @@ struct sift_bf_index {
     sift::PackedSet base;
     uint32_t r_n;
     int kind;
+    bool coarse = false;  // F32 over data that is not all integers in [0,255]
 };
+namespace sift {
+// Pack q_n queries into qset (work: sized in advance, no allocation) and run
+// the index's fused pass; flag = device int for the non-integer test.
+int bf_index_run(sift_context* h, sift_bf_index* ix, const void* q, int type, uint32_t q_n, const Knn2Out& out,
+                 int* flag, PackedSet& qset, const Knn2Work* work);
+}  // namespace sift
 
 struct sift_context {
     int device = 0;
@@ -169,7 +219,8 @@ struct sift_context {
     sift::DevBuf part;      // fused-kernel partial top-2
     sift::DevBuf thresh;    // per-query admission bounds from the seeding pass
     sift::DevBuf cand;      // FP8 coarse pass: 16 candidate rows per query
-    sift::DevBuf flag;      // int flags
+    sift::DevBuf flag;      // int[4]: [0] input outside the exact domain (cleared per call),
+                            // [2] a kernel's bounded stage wait gave up (sticky until read)
     sift::DevBuf stage_q, stage_r;  // host-input staging
 
     // profiling
@@ -182,6 +233,10 @@ struct sift_context {
     };
     Pending* pend = nullptr;
     int npend = 0, cap_pend = 0;
+
+    // poll limit of the fused kernel's stage hand-off waits (SIFT_WAIT_POLLS at
+    // creation; about 64 cycles per poll)
+    int wait_polls = 1 << 16;
 
     // launch shape picked by timing, per compute kind (index among the
     // kind's listed shapes; -1 = not yet timed)

@@ -28,7 +28,216 @@ struct MergeArgs {
     uint32_t* matches;
     float ratio;
     int* bound_out;       // seeding mode: write the shared 2nd-best bound instead
+    const int* coarse;    // float kinds: nonzero -> coarse keys (see Knn2Out::coarse)
+    float slack;          // coarse keys: relative rounding bound of the kind
 };
+
+// Coarse keys (float kinds on data that is not all integers in [0,255]):
+// with each value rounded to the kind with relative error at most u (2^-8
+// for BF16, 2^-11 for FP16, 0 for FP32), a key |b|^2 - 2 q.b of the MFMA pass
+// differs from the exact one by at most (2u + u^2) (|b|^2 + 2 |q| |b|) plus
+// the FP32 accumulation error of its 130 terms (130 * 2^-24 of their
+// magnitudes), i.e. by at most rel * (|q| + |b|)^2 with rel these values
+// (the bounds with a margin of about 2x).
+constexpr float COARSE_REL_BF16 = 1.f / 64.f;
+constexpr float COARSE_REL_F16 = 1.f / 512.f;
+constexpr float COARSE_REL_F32 = 1.f / 65536.f;
+inline float coarse_rel(int kind) {
+    return kind == KIND_BF16 ? COARSE_REL_BF16 : (kind == KIND_F16 ? COARSE_REL_F16 : COARSE_REL_F32);
+}
+
+// Candidates of the exact re-rank of coarse keys, per query.
+constexpr int EXACT_CAND = 16;
+
+// Eight consecutive dims [8c, 8c + 8) of row `row` as floats (exact
+// conversions): input rows of type 1..4, or an F32 packed set (group c,
+// lane half h holds dims 8c + 2i + h, see pack.hip).
+__device__ inline void load8_src(const void* p, int type, size_t row, int c, float v[8]) {
+    switch (type) {
+        case SIFT_FLOAT_DEVICE: {
+            const float4* s = reinterpret_cast<const float4*>(static_cast<const float*>(p) + row * 128 + c * 8);
+            const float4 a = s[0], b = s[1];
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            break;
+        }
+        case SIFT_HALF_DEVICE: {
+            const uint4 r = *reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(p) + row * 128 + c * 8);
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                _Float16 lo, hi;
+                const uint16_t l16 = (uint16_t)(w[i] & 0xFFFF), h16 = (uint16_t)(w[i] >> 16);
+                __builtin_memcpy(&lo, &l16, 2);
+                __builtin_memcpy(&hi, &h16, 2);
+                v[2 * i] = (float)lo;
+                v[2 * i + 1] = (float)hi;
+            }
+            break;
+        }
+        case SIFT_BF16_DEVICE: {
+            const uint4 r = *reinterpret_cast<const uint4*>(static_cast<const uint16_t*>(p) + row * 128 + c * 8);
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                v[2 * i] = __uint_as_float(w[i] << 16);
+                v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+            }
+            break;
+        }
+        case SIFT_U8_DEVICE: {
+            const uint2 r = *reinterpret_cast<const uint2*>(static_cast<const uint8_t*>(p) + row * 128 + c * 8);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                v[i] = (float)((r.x >> (8 * i)) & 0xFF);
+                v[4 + i] = (float)((r.y >> (8 * i)) & 0xFF);
+            }
+            break;
+        }
+        case SRC_BF16_TILES:
+        case SRC_F16_TILES: {  // 9 groups of 64 x 16 B per 32-row tile; dims 8c.. in group c/2, half c%2
+            const uint4 r = static_cast<const uint4*>(p)[(row >> 5) * 9 * 64 + (c >> 1) * 64 + (row & 31) + 32 * (c & 1)];
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (type == SRC_BF16_TILES) {
+                    v[2 * i] = __uint_as_float(w[i] << 16);
+                    v[2 * i + 1] = __uint_as_float(w[i] & 0xFFFF0000u);
+                } else {
+                    _Float16 lo, hi;
+                    const uint16_t l16 = (uint16_t)(w[i] & 0xFFFF), h16 = (uint16_t)(w[i] >> 16);
+                    __builtin_memcpy(&lo, &l16, 2);
+                    __builtin_memcpy(&hi, &h16, 2);
+                    v[2 * i] = (float)lo;
+                    v[2 * i + 1] = (float)hi;
+                }
+            }
+            break;
+        }
+        default: {  // SRC_F32_TILES: 17 groups of 64 x 16 B per 32-row tile
+            const uint4* t = static_cast<const uint4*>(p) + (row >> 5) * 17 * 64 + c * 64 + (row & 31);
+            const uint4 e = t[0], o = t[32];
+            v[0] = __uint_as_float(e.x); v[2] = __uint_as_float(e.y);
+            v[4] = __uint_as_float(e.z); v[6] = __uint_as_float(e.w);
+            v[1] = __uint_as_float(o.x); v[3] = __uint_as_float(o.y);
+            v[5] = __uint_as_float(o.z); v[7] = __uint_as_float(o.w);
+            break;
+        }
+    }
+}
+
+// host_lenght (src/knn_brute.cu:873-885): the squared distance as a
+// sequential FP32 sum over dims 0..127, one rounding per subtraction,
+// product and addition (no contraction).
+__device__ inline float seq_dist(const void* q, int qt, size_t qrow, const void* r, int rt, size_t rrow) {
+#pragma clang fp contract(off)
+    float d = 0.f;
+    for (int c = 0; c < 16; c++) {
+        float a[8], b[8];
+        load8_src(q, qt, qrow, c, a);
+        load8_src(r, rt, rrow, c, b);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const float e = a[i] - b[i];
+            d = d + e * e;
+        }
+    }
+    return d;
+}
+
+struct RerankArgs {
+    const int* coarse;
+    const uint4* part;
+    uint32_t q_n, q_stride;
+    int n_slabs;
+    ExactSrc src;
+    uint32_t* idx;
+    float* dist;
+    sift_float4* sorted;
+    uint32_t* matches;
+    float ratio;
+};
+
+// Exact re-rank of coarse keys, one wave per query (nothing to do when the
+// data was all integers in [0,255]): the EXACT_CAND smallest coarse (key,
+// row) over every slab partial (each lane a sorted list of its own entries,
+// then wave-wide minimum rounds), the sequential FP32 distance of each
+// candidate (one lane each), and the two smallest (distance, row) - the
+// result of host_brute restricted to the candidates.
+__global__ __launch_bounds__(256) void exact_rerank(RerankArgs a) {
+    if (*a.coarse == 0) return;
+    constexpr uint64_t NONE = ~0ull;
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= a.q_n) return;  // whole wave
+    uint64_t l[EXACT_CAND];
+#pragma unroll
+    for (int j = 0; j < EXACT_CAND; j++) l[j] = NONE;
+    auto push = [&](uint32_t key_bits, uint32_t row) {
+        if (row == 0xFFFFFFFFu) return;
+        const int o = Key<KIND_F32>::ord(__uint_as_float(key_bits));
+        uint64_t x = ((uint64_t)((uint32_t)o ^ 0x80000000u) << 32) | row;
+#pragma unroll
+        for (int j = 0; j < EXACT_CAND; j++) {  // insertion: l stays ascending
+            const uint64_t lo = min(l[j], x);
+            x = max(l[j], x);
+            l[j] = lo;
+        }
+    };
+    for (int s = lane; s < a.n_slabs; s += 64) {
+        const uint4 p = a.part[(size_t)s * a.q_stride + q];
+        push(p.x, p.y);
+        push(p.z, p.w);
+    }
+    uint32_t cand = 0xFFFFFFFFu;
+#pragma unroll
+    for (int r = 0; r < EXACT_CAND; r++) {
+        uint64_t m = l[0];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) m = min(m, __shfl_xor(m, off));
+        if (lane == r) cand = m == NONE ? 0xFFFFFFFFu : (uint32_t)m;
+        const bool pop = l[0] == m && m != NONE;  // rows are unique: one lane holds m
+#pragma unroll
+        for (int j = 0; j < EXACT_CAND - 1; j++) l[j] = pop ? l[j + 1] : l[j];
+        l[EXACT_CAND - 1] = pop ? NONE : l[EXACT_CAND - 1];
+    }
+    // distances are >= +0, so their bit patterns order as unsigned integers
+    uint64_t m1 = NONE, m2 = NONE;
+    if (lane < EXACT_CAND && cand != 0xFFFFFFFFu) {
+        const size_t qrow = a.src.q_map ? (size_t)a.src.q_map[q] : q;
+        const float d = seq_dist(a.src.q, a.src.q_type, qrow, a.src.r, a.src.r_type, cand);
+        m1 = ((uint64_t)__float_as_uint(d) << 32) | cand;
+    }
+#pragma unroll
+    for (int off = 1; off < EXACT_CAND; off <<= 1) {
+        const uint64_t b1 = __shfl_xor(m1, off, EXACT_CAND), b2 = __shfl_xor(m2, off, EXACT_CAND);
+        const uint64_t n2 = min(max(m1, b1), min(m2, b2));  // 2nd smallest of two sorted pairs
+        m1 = min(m1, b1);
+        m2 = n2;
+    }
+    if (lane != 0) return;
+    const bool h1 = m1 != NONE, h2 = m2 != NONE;
+    const float d1 = h1 ? __uint_as_float((uint32_t)(m1 >> 32)) : FLT_MAX;
+    const float d2 = h2 ? __uint_as_float((uint32_t)(m2 >> 32)) : FLT_MAX;
+    const uint32_t i1 = h1 ? (uint32_t)m1 : 0xFFFFFFFFu, i2 = h2 ? (uint32_t)m2 : 0xFFFFFFFFu;
+    if (a.idx) {
+        a.idx[2 * (size_t)q] = i1;
+        a.idx[2 * (size_t)q + 1] = i2;
+    }
+    if (a.dist) {
+        a.dist[2 * (size_t)q] = d1;
+        a.dist[2 * (size_t)q + 1] = d2;
+    }
+    if (a.sorted) {
+        sift_float4 o;
+        o.x = d1;
+        o.y = d2;
+        o.z = h1 ? (float)i1 : FLT_MAX;
+        o.w = h2 ? (float)i2 : FLT_MAX;
+        a.sorted[q] = o;
+    }
+    if (a.matches) a.matches[q] = (h1 && sqrtf(d1) / sqrtf(d2) < a.ratio) ? i1 : 0xFFFFFFFFu;
+}
 
 // Lexicographic merge of the slab partials, then d = key + ||q||^2 and the
 // output formats.  A block takes 64 queries; wave w merges slabs w, w + 8, ...
@@ -87,6 +296,8 @@ template <int KIND>
 __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
     __shared__ uint4 red[MERGE_WAVES - 1][64];
+    const bool coarse = a.coarse != nullptr && *a.coarse != 0;
+    if (coarse && !a.bound_out) return;  // the exact re-rank writes the outputs
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t q = blockIdx.x * 64u + lane;
     key_t m1 = Key<KIND>::none(), m2 = Key<KIND>::none();
@@ -114,6 +325,15 @@ __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
             // coarse keys: widen by FP8_SLACK of the coarse distance so rows
             // whose e4m3 rounding pushed them past the bound still enter
             if (has2) b = b + FP8_SLACK * fabsf(b + a.q_norm[q]) + 1.f;
+        } else if constexpr (KIND != KIND_I8) {
+            // coarse keys: the 2nd-best exact key is at most b + e and a true
+            // top-2 row's coarse key at most that + e, e = rel (|q| + |b|)^2
+            // with |b| <= |q| + sqrt(d2)
+            if (coarse && has2) {
+                const float nq = fmaxf(a.q_norm[q], 0.f);
+                const float s = 2.f * sqrtf(nq) + sqrtf(fmaxf(b + nq, 0.f));
+                b = b + 2.f * a.slack * s * s;
+            }
         }
         a.bound_out[q] = Key<KIND>::ord(b);
         return;
@@ -208,31 +428,61 @@ void choose_slabs(int n_tiles, int n_groups, int cap, int tps, int* n_slabs, int
     *tps_slab = std::max(tps, ((n_tiles + best_ns - 1) / best_ns + tps - 1) / tps * tps);
 }
 
-int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& qs,
-                       const PackedSet& bs, uint32_t q_n, uint32_t r_n, const Knn2Out& out) {
-    const int qpg = sh.qw * sh.waves * 32;
-    const int n_groups = (int)((q_n + qpg - 1) / qpg);
-    if ((uint64_t)n_groups * qpg > qs.rows_pad)
-        return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
-    const int n_tiles = (int)bs.ntiles;
-    const uint32_t q_stride = (uint32_t)n_groups * qpg;
-    const int cap = device_cu_count() * sh.blocks_per_cu();
-
-    // launch l covers tiles [t0[l], t0[l + 1]); its partials start at slab p0[l]
-    constexpr int MAX_LAUNCH = 8;
+// The seeded chain of one launch shape over a packed base: launch l covers
+// tiles [t0[l], t0[l + 1]) and writes its partials from slab p0[l].
+struct ChainPlan {
+    static constexpr int MAX_LAUNCH = 8;
+    int n_launch = 0, n_groups = 0;
+    uint32_t q_stride = 0;
     int t0[MAX_LAUNCH + 1], slabs[MAX_LAUNCH], tps_slab[MAX_LAUNCH], p0[MAX_LAUNCH + 1];
-    int n_launch = 1 + (r_n >= 2 ? seed_ends(n_tiles, t0 + 1, MAX_LAUNCH - 1) : 0);
-    t0[0] = 0;
-    t0[n_launch] = n_tiles;
-    p0[0] = 0;
-    for (int l = 0; l < n_launch; l++) {
-        choose_slabs(t0[l + 1] - t0[l], n_groups, cap, sh.tps, &slabs[l], &tps_slab[l]);
-        p0[l + 1] = p0[l] + slabs[l];
+    size_t part_bytes() const { return (size_t)p0[n_launch] * q_stride * sizeof(uint4); }
+    size_t bound_bytes() const { return (size_t)q_stride * sizeof(int); }
+};
+
+ChainPlan plan_chain(const FusedShape& sh, uint32_t n_tiles, uint32_t q_n, uint32_t r_n) {
+    ChainPlan c;
+    const int qpg = sh.qw * sh.waves * 32;
+    c.n_groups = (int)((q_n + qpg - 1) / qpg);
+    c.q_stride = (uint32_t)c.n_groups * qpg;
+    const int cap = device_cu_count() * sh.blocks_per_cu();
+    c.n_launch = 1 + (r_n >= 2 ? seed_ends((int)n_tiles, c.t0 + 1, ChainPlan::MAX_LAUNCH - 1) : 0);
+    c.t0[0] = 0;
+    c.t0[c.n_launch] = (int)n_tiles;
+    c.p0[0] = 0;
+    for (int l = 0; l < c.n_launch; l++) {
+        choose_slabs(c.t0[l + 1] - c.t0[l], c.n_groups, cap, sh.tps, &c.slabs[l], &c.tps_slab[l]);
+        c.p0[l + 1] = c.p0[l] + c.slabs[l];
     }
-    SIFT_TRY(reserve(h, h->part, (size_t)p0[n_launch] * q_stride * sizeof(uint4)));
-    SIFT_TRY(reserve(h, h->thresh, (size_t)q_stride * sizeof(int)));
-    uint4* part = static_cast<uint4*>(h->part.p);
-    int* bound = static_cast<int*>(h->thresh.p);
+    return c;
+}
+
+int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& qs,
+                       const PackedSet& bs, uint32_t q_n, uint32_t r_n, const Knn2Out& out,
+                       const Knn2Work* work) {
+    const ChainPlan c = plan_chain(sh, bs.ntiles, q_n, r_n);
+    if (c.q_stride > qs.rows_pad)
+        return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
+    const int n_groups = c.n_groups, n_launch = c.n_launch;
+    const uint32_t q_stride = c.q_stride;
+    const int* t0 = c.t0;
+    const int* slabs = c.slabs;
+    const int* tps_slab = c.tps_slab;
+    const int* p0 = c.p0;
+    uint4* part = nullptr;
+    int* bound = nullptr;
+    if (work) {  // caller scratch: no allocation
+        if (work->part_bytes < c.part_bytes() || work->bound_bytes < c.bound_bytes())
+            return set_error(h, SIFT_ERR_INVALID, "knn2 workspace smaller than the launch needs");
+        part = static_cast<uint4*>(work->part);
+        bound = work->bound;
+    } else {
+        SIFT_TRY(reserve(h, h->part, c.part_bytes()));
+        SIFT_TRY(reserve(h, h->thresh, c.bound_bytes()));
+        part = static_cast<uint4*>(h->part.p);
+        bound = static_cast<int*>(h->thresh.p);
+    }
+    const bool float_kind = bs.kind == KIND_BF16 || bs.kind == KIND_F16 || bs.kind == KIND_F32;
+    const int* coarse = float_kind ? out.coarse : nullptr;
 
     auto mk_fused = [&](int l) {
         FusedArgs fa{};
@@ -251,6 +501,9 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         fa.tiles_per_slab = tps_slab[l];
         fa.n_slabs = slabs[l];
         fa.n_groups = n_groups;
+        fa.wait_polls = h->wait_polls;
+        fa.status = wait_status(h);
+        fa.coarse = coarse;
         return fa;
     };
     auto mk_merge = [&](int n_part) {
@@ -267,6 +520,8 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         ma.matches = out.matches;
         ma.ratio = out.ratio;
         ma.bound_out = nullptr;
+        ma.coarse = coarse;
+        ma.slack = coarse_rel(bs.kind);
         return ma;
     };
     auto launch_merge = [&](const MergeArgs& ma) {
@@ -308,6 +563,25 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         launch_merge(mk_merge(p0[n_launch]));
         SIFT_HIP_TRY(h, hipGetLastError());
     }
+    if (coarse) {  // does nothing on the device when the data was all integers in [0,255]
+        if (!out.src.q || (r_n && !out.src.r))
+            return set_error(h, SIFT_ERR_INVALID, "coarse keys need the rows for the exact re-rank");
+        ProfScope ps(h, PROF_MERGE);
+        RerankArgs ra;
+        ra.coarse = coarse;
+        ra.part = part;
+        ra.q_n = q_n;
+        ra.q_stride = q_stride;
+        ra.n_slabs = p0[n_launch];
+        ra.src = out.src;
+        ra.idx = out.idx;
+        ra.dist = out.dist;
+        ra.sorted = out.sorted;
+        ra.matches = out.matches;
+        ra.ratio = out.ratio;
+        exact_rerank<<<(q_n + 3) / 4, 256, 0, h->stream>>>(ra);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     return SIFT_OK;
 }
 
@@ -321,19 +595,37 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
 constexpr uint32_t TUNE_MIN_TILES = 8192;
 constexpr int TUNE_ROUNDS = 3;
 
+const FusedShape* knn2_shape(sift_context* h, int kind, uint32_t n_tiles) {
+    const FusedShape* sh = pick_shape(kind);
+    if (!sh || std::getenv("SIFT_SHAPE") || n_tiles < TUNE_MIN_TILES || kind < 0 || kind >= 8) return sh;
+    const FusedShape* cand[16];
+    const int nc = std::min(2, shapes_of_kind(kind, cand, 16));
+    return (h->tuned_shape[kind] >= 0 && h->tuned_shape[kind] < nc) ? cand[h->tuned_shape[kind]] : sh;
+}
+
+size_t knn2_work_bytes(sift_context* h, int kind, uint32_t n_tiles, uint32_t q_n, uint32_t r_n, size_t* bound_bytes) {
+    const FusedShape* sh = knn2_shape(h, kind, n_tiles);
+    if (!sh) return 0;
+    const ChainPlan c = plan_chain(*sh, n_tiles, q_n, r_n);
+    if (bound_bytes) *bound_bytes = c.bound_bytes();
+    return c.part_bytes();
+}
+
 int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint32_t q_n,
-                uint32_t r_n, const Knn2Out& out) {
+                uint32_t r_n, const Knn2Out& out, const Knn2Work* work) {
     if (qs.kind != bs.kind) return set_error(h, SIFT_ERR_INVALID, "query/base packed with different kinds");
     if (q_n == 0) return SIFT_OK;
     const FusedShape* sh = pick_shape(bs.kind);
     if (!sh) return set_error(h, SIFT_ERR_INVALID, "bad kind");
     const int kind = bs.kind;
+    // caller scratch: the shape already tuned on this handle, or the default; never timed here
+    if (work) return launch_fused_shape(h, *knn2_shape(h, kind, bs.ntiles), qs, bs, q_n, r_n, out, work);
     if (std::getenv("SIFT_SHAPE") || bs.ntiles < TUNE_MIN_TILES || kind < 0 || kind >= 8)
-        return launch_fused_shape(h, *sh, qs, bs, q_n, r_n, out);
+        return launch_fused_shape(h, *sh, qs, bs, q_n, r_n, out, nullptr);
     const FusedShape* cand[16];
     const int nc = std::min(2, shapes_of_kind(kind, cand, 16));
     if (h->tuned_shape[kind] >= 0 && h->tuned_shape[kind] < nc)
-        return launch_fused_shape(h, *cand[h->tuned_shape[kind]], qs, bs, q_n, r_n, out);
+        return launch_fused_shape(h, *cand[h->tuned_shape[kind]], qs, bs, q_n, r_n, out, nullptr);
     hipEvent_t e[2];
     SIFT_HIP_TRY(h, hipEventCreate(&e[0]));
     if (hipEventCreate(&e[1]) != hipSuccess) {
@@ -348,7 +640,7 @@ int knn2_packed(sift_context* h, const PackedSet& qs, const PackedSet& bs, uint3
     for (int r = 0; r < TUNE_ROUNDS && rc == SIFT_OK; r++) {
         for (int i = 0; i < nc && rc == SIFT_OK; i++) {
             (void)hipEventRecord(e[0], h->stream);
-            rc = launch_fused_shape(h, *cand[i], qs, bs, q_n, r_n, out);
+            rc = launch_fused_shape(h, *cand[i], qs, bs, q_n, r_n, out, nullptr);
             (void)hipEventRecord(e[1], h->stream);
             if (rc != SIFT_OK) break;
             float ms = 0.f;
@@ -416,7 +708,7 @@ int resolve_kind(sift_context* h, const void* q, const void* r, int type, uint32
 namespace {
 
 int knn2_impl(sift_context* h, const void* q, const void* r, int type, uint32_t q_n, uint32_t r_n,
-              int compute, int fallback, const Knn2Out& out, bool blocking) {
+              int compute, int fallback, const Knn2Out& out_in, bool blocking, const char* what) {
     if (!h) return set_error(nullptr, SIFT_ERR_INVALID, "null handle");
     if (!valid_type(type)) return set_error(h, SIFT_ERR_INVALID, "unknown input type " + std::to_string(type));
     if (q_n == 0) return SIFT_OK;
@@ -425,21 +717,27 @@ int knn2_impl(sift_context* h, const void* q, const void* r, int type, uint32_t 
     if (type == SIFT_FLOAT_HOST) {
         SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
         SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+        type = SIFT_FLOAT_DEVICE;
     }
     int kind = 0;
     SIFT_TRY(resolve_kind(h, q, r, type, q_n, r_n, compute, fallback, &kind));
-    SIFT_TRY(reserve(h, h->flag, 16));
     int* flag = static_cast<int*>(h->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 4, h->stream));
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
     SIFT_TRY(pack_side(h, r, type, r_n, rows_b, kind, PACK_BASE, h->b_set, flag));
     SIFT_TRY(pack_side(h, q, type, q_n, query_rows_pad(kind, q_n), kind, PACK_QUERY, h->q_set, flag));
+    // float kinds: data that is not all integers in [0,255] (flagged by the
+    // packs) is ranked coarsely and re-ranked exactly from the input rows
+    Knn2Out out = out_in;
+    out.coarse = flag;
+    out.src = ExactSrc{q, type, r, type};
     SIFT_TRY(knn2_packed(h, h->q_set, h->b_set, q_n, r_n, out));
     if (blocking) {
-        int bad = 0;
-        SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
+        int fl[3] = {0, 0, 0};
+        SIFT_HIP_TRY(h, hipMemcpyAsync(fl, flag, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
         SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-        if (bad && (kind == KIND_I8 || kind == KIND_FP8))
+        SIFT_TRY(wait_error(h, fl[2], what));
+        if (fl[0] && (kind == KIND_I8 || kind == KIND_FP8))
             return set_error(h, SIFT_ERR_NOT_EXACT, kind == KIND_I8 ? "SIFT_COMPUTE_I8 needs integer data in [0,255]"
                                                                     : "SIFT_COMPUTE_FP8 needs integer data in [0,255]");
     }
@@ -466,14 +764,14 @@ int sift_knn2(sift_handle_t h, const void* q, const void* r, int type, uint32_t 
               int compute, uint32_t* idx, float* dist, sift_float4* sorted, uint32_t* matches,
               float ratio) {
     Knn2Out o{idx, dist, sorted, matches, ratio};
-    return knn2_impl(h, q, r, type, q_n, r_n, compute, KIND_F32, o, true);
+    return knn2_impl(h, q, r, type, q_n, r_n, compute, KIND_F32, o, true, "sift_knn2");
 }
 
 int sift_knn2_async(sift_handle_t h, const void* q, const void* r, int type, uint32_t q_n,
                     uint32_t r_n, int compute, uint32_t* idx, float* dist, sift_float4* sorted,
                     uint32_t* matches, float ratio) {
     Knn2Out o{idx, dist, sorted, matches, ratio};
-    return knn2_impl(h, q, r, type, q_n, r_n, compute, KIND_F32, o, false);
+    return knn2_impl(h, q, r, type, q_n, r_n, compute, KIND_F32, o, false, "sift_knn2_async");
 }
 
 // src/knn_brute.cu:24-209.  fp16 semantics for non-integer data (the
@@ -496,7 +794,7 @@ int cublas_2nn_sift(void* q_points, void* r_points, int type, uint32_t q_n, uint
             DevBuf tmp;
             SIFT_TRY(reserve(handle, tmp, (size_t)q_n * 4));
             o.matches = static_cast<uint32_t*>(tmp.p);
-            int rc = knn2_impl(handle, q_points, r_points, type, q_n, r_n, compute, KIND_F16, o, true);
+            int rc = knn2_impl(handle, q_points, r_points, type, q_n, r_n, compute, KIND_F16, o, true, "cublas_2nn_sift");
             if (rc == SIFT_OK)
                 rc = hipMemcpy(matches, tmp.p, (size_t)q_n * 4, hipMemcpyDeviceToHost) == hipSuccess
                          ? SIFT_OK
@@ -506,7 +804,7 @@ int cublas_2nn_sift(void* q_points, void* r_points, int type, uint32_t q_n, uint
         }
     }
     o.matches = matches;
-    return knn2_impl(handle, q_points, r_points, type, q_n, r_n, compute, KIND_F16, o, true);
+    return knn2_impl(handle, q_points, r_points, type, q_n, r_n, compute, KIND_F16, o, true, "cublas_2nn_sift");
 }
 
 // src/knn_brute.cu:213-278.  R is fp16 on the device; Q and dist are unused.
@@ -519,29 +817,31 @@ int cublas_2nn_sift_batch(void* q_points, void* Q, int type, void* R, uint32_t q
     if (!handle) return set_error(nullptr, SIFT_ERR_INVALID, "null handle");
     if (type != SIFT_HALF_DEVICE && type != SIFT_FLOAT_DEVICE)
         return set_error(handle, SIFT_ERR_UNSUPPORTED, "cublas_2nn_sift_batch: q must be float or half on the device");
+    if (q_n == 0) return SIFT_OK;
+    if (!q_points || (r_n && !R) || !matches) return set_error(handle, SIFT_ERR_INVALID, "null pointer");
+    if (r_n >= (1u << 24)) return set_error(handle, SIFT_ERR_UNSUPPORTED, "r_n must be < 2^24");
+    // work queued on the caller's stream (e.g. the conversion of R) comes first
     if (stream && stream != (void*)handle->stream)
         SIFT_HIP_TRY(handle, hipStreamSynchronize(static_cast<hipStream_t>(stream)));
-    const int compute = SIFT_COMPUTE_F16;
-    // R arrives as fp16: pack it as such, pack q from its own type
-    if (!handle || q_n == 0) return SIFT_OK;
-    SIFT_TRY(reserve(handle, handle->flag, 16));
+    // R arrives as fp16: FP16 MFMAs over R and q rounded to fp16; data that is
+    // not all integers in [0,255] is re-ranked exactly from R and q as given
     int* flag = static_cast<int*>(handle->flag.p);
+    SIFT_HIP_TRY(handle, hipMemsetAsync(flag, 0, 4, handle->stream));
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
     SIFT_TRY(pack_rows(handle, R, SIFT_HALF_DEVICE, r_n, rows_b, KIND_F16, PACK_BASE, handle->b_set, flag));
     SIFT_TRY(pack_rows(handle, q_points, type, q_n, query_rows_pad(KIND_F16, q_n), KIND_F16, PACK_QUERY,
                        handle->q_set, flag));
-    (void)compute;
     Knn2Out o;
     o.matches = matches;
     o.ratio = threshold;
+    o.coarse = flag;
+    o.src = ExactSrc{q_points, type, R, SIFT_HALF_DEVICE};
     SIFT_TRY(knn2_packed(handle, handle->q_set, handle->b_set, q_n, r_n, o));
-    SIFT_HIP_TRY(handle, hipStreamSynchronize(handle->stream));
-    return SIFT_OK;
+    return check_wait(handle, "cublas_2nn_sift_batch");
 }
 
 // src/knn_brute.cu:619-667: exact FP32 2-NN into float4.
-void device_brute(const float* q_points, const float* r_points, int q_n, int r_n,
-                  sift_float4* sorted) {
+void device_brute(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted) {
     sift_context* h = default_handle();
     if (!h) {
         set_error(nullptr, SIFT_ERR_HIP, "device_brute: no default handle");
@@ -552,15 +852,15 @@ void device_brute(const float* q_points, const float* r_points, int q_n, int r_n
         return;
     }
     Knn2Out o;
-    o.sorted = sorted;
+    o.sorted = static_cast<sift_float4*>(sorted);
     int rc = knn2_impl(h, q_points, r_points, SIFT_FLOAT_DEVICE, (uint32_t)q_n, (uint32_t)r_n,
-                       SIFT_COMPUTE_AUTO, KIND_F32, o, true);
+                       SIFT_COMPUTE_AUTO, KIND_F32, o, true, "device_brute");
     if (rc == SIFT_OK) set_error(h, SIFT_OK, "");
 }
 
 // src/knn_brute.cu:523-563.
-void cublas_2nn_f(const float* q_points, const float* r_points, int q_n, int r_n,
-                  sift_float4* sorted, sift_handle_t handle) {
+void cublas_2nn_f(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted,
+                  sift_handle_t handle) {
     if (!handle) {
         set_error(nullptr, SIFT_ERR_INVALID, "cublas_2nn_f: null handle");
         return;
@@ -570,15 +870,15 @@ void cublas_2nn_f(const float* q_points, const float* r_points, int q_n, int r_n
         return;
     }
     Knn2Out o;
-    o.sorted = sorted;
+    o.sorted = static_cast<sift_float4*>(sorted);
     int rc = knn2_impl(handle, q_points, r_points, SIFT_FLOAT_DEVICE, (uint32_t)q_n, (uint32_t)r_n,
-                       SIFT_COMPUTE_AUTO, KIND_F32, o, true);
+                       SIFT_COMPUTE_AUTO, KIND_F32, o, true, "cublas_2nn_f");
     if (rc == SIFT_OK) set_error(handle, SIFT_OK, "");
 }
 
 // src/knn_brute.cu:567-616 (dist buffer unused).
-void cublas_2nn_brute_f(const float* q_points, const float* r_points, int q_n, int r_n,
-                        sift_float4* sorted, float* dist, sift_handle_t handle) {
+void cublas_2nn_brute_f(const des_t_f* q_points, const des_t_f* r_points, int q_n, int r_n, void* sorted,
+                        float* dist, sift_handle_t handle) {
     (void)dist;
     cublas_2nn_f(q_points, r_points, q_n, r_n, sorted, handle);
 }
@@ -593,27 +893,34 @@ int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, 
     if (!h || !index) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
     if (!valid_type(type)) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
     if (r_n >= (1u << 24)) return set_error(h, SIFT_ERR_UNSUPPORTED, "r_n must be < 2^24");
+    if (r_n > 0 && !r) return set_error(h, SIFT_ERR_INVALID, "null base");
     *index = nullptr;
-    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+        type = SIFT_FLOAT_DEVICE;
+    }
     int kind = 0;
     SIFT_TRY(resolve_kind(h, r, r, type, 0, r_n, compute, KIND_F32, &kind));
     auto* ix = new sift_bf_index();
     ix->h = h;
     ix->r_n = r_n;
     ix->kind = kind;
-    SIFT_TRY(reserve(h, h->flag, 16));
     int* flag = static_cast<int*>(h->flag.p);
-    (void)hipMemsetAsync(flag, 0, 4, h->stream);
+    int rc = hipMemsetAsync(flag, 0, 4, h->stream) == hipSuccess ? SIFT_OK : set_error(h, SIFT_ERR_HIP, "memset");
     const uint32_t rows_b = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-    int rc = pack_side(h, r, type, r_n, rows_b, kind, PACK_BASE, ix->base, flag);
+    if (rc == SIFT_OK) rc = pack_side(h, r, type, r_n, rows_b, kind, PACK_BASE, ix->base, flag);
     if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess)
         rc = set_error(h, SIFT_ERR_HIP, "index build sync");
-    if (rc == SIFT_OK && (kind == KIND_I8 || kind == KIND_FP8)) {
+    if (rc == SIFT_OK) {
         int bad = 0;
-        if (hipMemcpy(&bad, flag, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipMemcpy(&bad, flag, 4, hipMemcpyDeviceToHost) != hipSuccess) {
             rc = set_error(h, SIFT_ERR_HIP, "index build flag");
-        else if (bad)
-            rc = set_error(h, SIFT_ERR_NOT_EXACT, "I8 and FP8 indexes need integer data in [0,255]");
+        } else if (bad && kind != KIND_F32) {
+            // the packed rows are all the index keeps: only FP32 holds other data exactly
+            rc = set_error(h, SIFT_ERR_NOT_EXACT,
+                           "BF16, F16, I8 and FP8 indexes need integer data in [0,255] (use F32 for other data)");
+        }
+        ix->coarse = bad != 0;  // F32 over other data: coarse keys, exact re-rank from the tiles
     }
     if (rc != SIFT_OK) {
         release_set(ix->base);
@@ -624,24 +931,50 @@ int sift_bf_index_build(sift_handle_t h, const void* r, int type, uint32_t r_n, 
     return SIFT_OK;
 }
 
+}  // extern "C"
+
+namespace sift {
+// Query an index: pack the queries (flag: device int, set when some query
+// value is not an integer in [0,255]), then the fused pass; float kinds
+// re-rank exactly from the packed base tiles (integer BF16/F16 bases, any
+// F32 base) and the query rows as given when the keys are coarse.
+int bf_index_run(sift_context* h, sift_bf_index* ix, const void* q, int type, uint32_t q_n, const Knn2Out& out_in,
+                 int* flag, PackedSet& qset, const Knn2Work* work) {
+    SIFT_HIP_TRY(h, hipMemsetAsync(flag, ix->coarse ? 1 : 0, 4, h->stream));
+    SIFT_TRY(work ? pack_launch(h, q, type, q_n, nullptr, ix->kind, PACK_QUERY, qset, flag)
+                  : pack_side(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, qset, flag));
+    Knn2Out o = out_in;
+    if (ix->kind == KIND_BF16 || ix->kind == KIND_F16 || ix->kind == KIND_F32) {
+        o.coarse = flag;
+        o.src = ExactSrc{q, type, ix->base.tiles.p,
+                         ix->kind == KIND_F32 ? SRC_F32_TILES : (ix->kind == KIND_BF16 ? SRC_BF16_TILES : SRC_F16_TILES)};
+    }
+    return knn2_packed(h, qset, ix->base, q_n, ix->r_n, o, work);
+}
+}  // namespace sift
+
+extern "C" {
+
 int sift_bf_index_query(sift_handle_t h, sift_bf_index_t ix, const void* q, int type, uint32_t q_n,
                         uint32_t* idx, float* dist, sift_float4* sorted, uint32_t* matches,
                         float ratio, int blocking) {
     if (!h || !ix) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
     if (!valid_type(type)) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
     if (q_n == 0) return SIFT_OK;
-    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
-    SIFT_TRY(reserve(h, h->flag, 16));
+    if (!q) return set_error(h, SIFT_ERR_INVALID, "null queries");
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
+        type = SIFT_FLOAT_DEVICE;
+    }
     int* flag = static_cast<int*>(h->flag.p);
-    SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 4, h->stream));
-    SIFT_TRY(pack_side(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, h->q_set, flag));
     Knn2Out o{idx, dist, sorted, matches, ratio};
-    SIFT_TRY(knn2_packed(h, h->q_set, ix->base, q_n, ix->r_n, o));
+    SIFT_TRY(bf_index_run(h, ix, q, type, q_n, o, flag, h->q_set, nullptr));
     if (blocking) {
-        int bad = 0;
-        SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
+        int fl[3] = {0, 0, 0};
+        SIFT_HIP_TRY(h, hipMemcpyAsync(fl, flag, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
         SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-        if (bad && (ix->kind == KIND_I8 || ix->kind == KIND_FP8))
+        SIFT_TRY(wait_error(h, fl[2], "sift_bf_index_query"));
+        if (fl[0] && (ix->kind == KIND_I8 || ix->kind == KIND_FP8))
             return set_error(h, SIFT_ERR_NOT_EXACT, "I8 and FP8 indexes need integer queries in [0,255]");
     }
     return SIFT_OK;

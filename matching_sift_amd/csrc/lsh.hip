@@ -992,6 +992,8 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     uint32_t cnt = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    // the LSH pass's own input check (the non-blocking query skipped it)
+    SIFT_TRY(check_flag(h, static_cast<int*>(sc->flag.p), "sift_lsh_query_fallback"));
     if (n_fallback) *n_fallback = cnt;
     if (cnt == 0) return SIFT_OK;
     const uint32_t rows_pad = query_rows_pad(bf->kind, cnt);
@@ -1007,20 +1009,23 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     qs.rows_pad = rows_pad;
     qs.ntiles = rows_pad / 32;
     qs.kind = bf->kind;
-    SIFT_TRY(reserve(h, sc->flag, 16));
-    SIFT_TRY(pack_launch(h, q, type, 0, static_cast<const int32_t*>(sc->fb_map.p), bf->kind, PACK_QUERY, qs,
-                         static_cast<int*>(sc->flag.p)));
+    int* flag = static_cast<int*>(sc->flag.p);
+    SIFT_HIP_TRY(h, hipMemsetAsync(flag, bf->coarse ? 1 : 0, 4, h->stream));
+    const auto* fb_map = static_cast<const int32_t*>(sc->fb_map.p);
+    SIFT_TRY(pack_launch(h, q, type, 0, fb_map, bf->kind, PACK_QUERY, qs, flag));
     SIFT_TRY(reserve(h, sc->fb_out, (size_t)cnt * 20));
     auto* oi = static_cast<uint32_t*>(sc->fb_out.p);
     auto* od = reinterpret_cast<float*>(oi + 2 * (size_t)cnt);
     auto* om = oi + 4 * (size_t)cnt;
     Knn2Out o{oi, od, nullptr, om, ratio};
+    if (bf->kind == KIND_F32) {  // an F32 index over other data: exact re-rank from its tiles
+        o.coarse = flag;
+        o.src = ExactSrc{q, type, bf->base.tiles.p, SRC_F32_TILES, fb_map};
+    }
     SIFT_TRY(knn2_packed(h, qs, bf->base, cnt, bf->r_n, o));
-    scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(static_cast<const int32_t*>(sc->fb_map.p), cnt, oi, od,
-                                                            om, idx, dist, matches);
+    scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(fb_map, cnt, oi, od, om, idx, dist, matches);
     SIFT_HIP_TRY(h, hipGetLastError());
-    SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-    return SIFT_OK;
+    return check_wait(h, "sift_lsh_query_fallback");
 }
 
 // ------------------------------------------------------- persistence ----
@@ -1089,38 +1094,97 @@ int sift_lsh_save(sift_lsh_index_t ix, const char* path) {
     return ok ? SIFT_OK : set_error(h, SIFT_ERR_INVALID, std::string("write failed: ") + path);
 }
 
-int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* out) {
-    if (!h || !path || !out) return set_error(h, SIFT_ERR_INVALID, "null handle/path/index");
-    *out = nullptr;
-    FILE* f = std::fopen(path, "rb");
-    if (!f) return set_error(h, SIFT_ERR_INVALID, std::string("cannot open ") + path);
-    struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+}  // extern "C"
+
+namespace {
+// An index file read into host memory and checked.
+struct LshFile {
     LshFileHeader hd{};
+    std::vector<float> hash;
+    std::vector<uint32_t> keys, vals;
+    std::vector<uint8_t> u8;
+    std::vector<int32_t> nrm;
+};
+
+// Read and check an index file without touching the device.  The contents
+// must be what a build writes before anything reaches the device: the re-rank
+// gathers rows by the stored ids and the bucket lookups assume each table's
+// keys sorted under its own prefix.  Returns an empty string or the reason.
+std::string read_lsh_file(const char* path, LshFile& lf) {
+    const std::string p(path);
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return "cannot open " + p;
+    struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
+    LshFileHeader& hd = lf.hd;
     if (std::fread(&hd, sizeof hd, 1, f) != 1 || std::memcmp(hd.magic, LSH_MAGIC, 8) != 0 || hd.version != 1)
-        return set_error(h, SIFT_ERR_INVALID, std::string(path) + " is not an LSH index file");
+        return p + " is not an LSH index file";
     int tbits = 0;
     while (hd.L > 0 && (1 << tbits) < hd.L) tbits++;
     const bool learned = hd.hash_kind == SIFT_LSH_LEARNED;
     if (hd.nbits < 1 || hd.nbits > 32 || hd.L < 1 || hd.L > 256 || hd.nbits + tbits > 32 ||
-        (uint64_t)hd.n * (uint64_t)hd.L >= (1ull << 32) ||
+        (uint64_t)hd.n * (uint64_t)hd.L >= (1ull << 32) || hd.n >= (1u << 24) ||
         (hd.hash_kind != SIFT_LSH_HYPERPLANE && hd.hash_kind != SIFT_LSH_HYPERPLANE_PM1 && !learned) ||
-        (learned && hd.nparam != (uint32_t)SIFT_MLP_NPARAM))
-        return set_error(h, SIFT_ERR_INVALID, std::string(path) + ": bad index header");
+        (learned && hd.nparam != (uint32_t)SIFT_MLP_NPARAM) || (!learned && hd.nparam != 0))
+        return p + ": bad index header";
     std::fseek(f, 0, SEEK_END);
     const long size = std::ftell(f);
-    if (size < 0 || (size_t)size != lsh_file_bytes(hd))
-        return set_error(h, SIFT_ERR_INVALID, std::string(path) + ": size does not match its header");
+    if (size < 0 || (size_t)size != lsh_file_bytes(hd)) return p + ": size does not match its header";
     std::fseek(f, (long)sizeof hd, SEEK_SET);
-    std::vector<float> hash(learned ? (size_t)hd.L * hd.nparam : (size_t)hd.L * hd.nbits * 128);
+    lf.hash.resize(learned ? (size_t)hd.L * hd.nparam : (size_t)hd.L * hd.nbits * 128);
     const size_t nk = (size_t)hd.L * hd.n;
-    std::vector<uint32_t> keys(nk), vals(nk);
-    std::vector<uint8_t> u8((size_t)hd.n * 128);
-    std::vector<int32_t> nrm(hd.n);
-    bool ok = std::fread(hash.data(), 4, hash.size(), f) == hash.size();
-    ok = ok && (nk == 0 || (std::fread(keys.data(), 4, nk, f) == nk && std::fread(vals.data(), 4, nk, f) == nk));
-    ok = ok && (hd.n == 0 || (std::fread(u8.data(), 1, u8.size(), f) == u8.size() &&
-                              std::fread(nrm.data(), 4, nrm.size(), f) == nrm.size()));
-    if (!ok) return set_error(h, SIFT_ERR_INVALID, std::string("short read: ") + path);
+    lf.keys.resize(nk);
+    lf.vals.resize(nk);
+    lf.u8.resize((size_t)hd.n * 128);
+    lf.nrm.resize(hd.n);
+    bool ok = std::fread(lf.hash.data(), 4, lf.hash.size(), f) == lf.hash.size();
+    ok = ok && (nk == 0 || (std::fread(lf.keys.data(), 4, nk, f) == nk && std::fread(lf.vals.data(), 4, nk, f) == nk));
+    ok = ok && (hd.n == 0 || (std::fread(lf.u8.data(), 1, lf.u8.size(), f) == lf.u8.size() &&
+                              std::fread(lf.nrm.data(), 4, lf.nrm.size(), f) == lf.nrm.size()));
+    if (!ok) return "short read: " + p;
+    for (size_t t = 0; t < (size_t)hd.L; t++) {
+        const uint32_t prefix = hd.nbits < 32 ? (uint32_t)t : 0u;
+        for (size_t i = t * hd.n; i < (t + 1) * hd.n; i++) {
+            if (lf.vals[i] >= hd.n) return p + ": a bucket names a row past the base";
+            if ((hd.nbits < 32 ? lf.keys[i] >> hd.nbits : 0u) != prefix) return p + ": a key lies outside its table";
+            if (i > t * hd.n && lf.keys[i] < lf.keys[i - 1]) return p + ": the keys of a table are not sorted";
+        }
+    }
+    for (size_t j = 0; j < hd.n; j++) {
+        int32_t s = 0;
+        for (int d = 0; d < 128; d++) s += (int32_t)lf.u8[j * 128 + d] * lf.u8[j * 128 + d];
+        if (s != lf.nrm[j]) return p + ": a stored norm does not match its row";
+    }
+    if (!learned)
+        for (float v : lf.hash)
+            if (!(v >= -127.f && v <= 127.f && v == std::nearbyint(v)))
+                return p + ": a hyperplane value is not an integer in [-127, 127]";
+    return std::string();
+}
+}  // namespace
+
+extern "C" {
+
+int sift_lsh_check_file(const char* path) {
+    if (!path) return set_error(nullptr, SIFT_ERR_INVALID, "null path");
+    LshFile lf;
+    const std::string why = read_lsh_file(path, lf);
+    return why.empty() ? SIFT_OK : set_error(nullptr, SIFT_ERR_INVALID, why);
+}
+
+int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* out) {
+    if (!h || !path || !out) return set_error(h, SIFT_ERR_INVALID, "null handle/path/index");
+    *out = nullptr;
+    LshFile lf;
+    const std::string why = read_lsh_file(path, lf);
+    if (!why.empty()) return set_error(h, SIFT_ERR_INVALID, why);
+    const LshFileHeader& hd = lf.hd;
+    const bool learned = hd.hash_kind == SIFT_LSH_LEARNED;
+    const size_t nk = (size_t)hd.L * hd.n;
+    std::vector<float>& hash = lf.hash;
+    std::vector<uint32_t>& keys = lf.keys;
+    std::vector<uint32_t>& vals = lf.vals;
+    std::vector<uint8_t>& u8 = lf.u8;
+    std::vector<int32_t>& nrm = lf.nrm;
 
     auto* ix = new sift_lsh_index();
     ix->h = h;

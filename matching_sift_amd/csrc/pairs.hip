@@ -218,6 +218,8 @@ FusedArgs make_args(sift_pairs_plan* p, uint32_t* idx, float* dist, uint32_t* ma
     a.ratio = ratio;
     a.neg64 = 64;  // I8 composite = 64 q''.b' + (c << 5 | slot) = key * 32 + slot
     a.out_cand = static_cast<uint32_t*>(p->cand.p);
+    a.wait_polls = p->h->wait_polls;
+    a.status = wait_status(p->h);
     return a;
 }
 
@@ -527,7 +529,7 @@ int sift_pairs_plan_execute(sift_handle_t h, sift_pairs_plan_t p, const void* q,
     } else {
         SIFT_TRY(run_sequence(p, q, r, idx, dist, matches, ratio, true));
     }
-    if (blocking) SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    if (blocking) return check_wait(h, "sift_pairs_plan_execute");
     return SIFT_OK;
 }
 

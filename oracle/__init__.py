@@ -6,6 +6,7 @@ cpu_baseline leg of bench.py may import this package, and only as the checker
 
 Function -> reference:
   host_brute        src/knn_brute.cu:801-835 (+ host_lenght :873-885)
+  host_lenght       src/knn_brute.cu:873-885 (listed query/base pairs)
   host_sort         src/knn_brute.cu:839-870
   ratio             src/knn_brute.cu:430-442 (true L2, no unit-norm +2)
   lsh_codes         src/lsh_h.cu:265-317 (bit b set when h_b.x < 0)
@@ -37,6 +38,7 @@ def _lib():
         vp, i32, f32 = C.c_void_p, C.c_int, C.c_float
         sig = {
             "oracle_host_brute": [vp, vp, i32, i32, vp],
+            "oracle_host_lenght": [vp, vp, vp, vp, i32, vp],
             "oracle_host_brute_mt": [vp, vp, i32, i32, vp, i32],
             "oracle_host_brute_mutant": [vp, vp, i32, i32, vp, i32],
             "oracle_host_sort": [vp, i32, i32, vp],
@@ -74,6 +76,19 @@ def host_brute(q, r, threads=1):
     else:
         _lib().oracle_host_brute_mt(q.ctypes.data, r.ctypes.data, q.shape[0], r.shape[0],
                                     out.ctypes.data, threads)
+    return out
+
+
+def host_lenght(q, r, qi, ri):
+    """float32 [n]: host_lenght(q[qi[k]], r[ri[k]]) (src/knn_brute.cu:873-885)."""
+    q, r = _f32(q), _f32(r)
+    qi = np.ascontiguousarray(qi, np.int64)
+    ri = np.ascontiguousarray(ri, np.int64)
+    assert qi.shape == ri.shape and qi.ndim == 1
+    assert (qi >= 0).all() and (qi < q.shape[0]).all() and (ri >= 0).all() and (ri < r.shape[0]).all()
+    out = np.empty((qi.shape[0],), np.float32)
+    _lib().oracle_host_lenght(q.ctypes.data, r.ctypes.data, qi.ctypes.data, ri.ctypes.data, qi.shape[0],
+                              out.ctypes.data)
     return out
 
 

@@ -54,6 +54,11 @@ static void host_brute_one(const float* qi, const float* r, int r_n, float* out4
     out4[3] = w;
 }
 
+void oracle_host_lenght(const float* q, const float* r, const int64_t* qi, const int64_t* ri, int n,
+                        float* out) {
+    for (int i = 0; i < n; i++) out[i] = host_lenght(q + (size_t)qi[i] * 128, r + (size_t)ri[i] * 128);
+}
+
 void oracle_host_brute(const float* q, const float* r, int q_n, int r_n, float* sorted4) {
     for (int i = 0; i < q_n; i++) host_brute_one(q + (size_t)i * 128, r, r_n, sorted4 + (size_t)i * 4);
 }

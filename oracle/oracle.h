@@ -13,6 +13,10 @@
 extern "C" {
 #endif
 
+/* host_lenght (src/knn_brute.cu:873-885) of n (query, base row) pairs:
+ * out[i] = sequential FP32 sum of (q[qi[i]] - r[ri[i]])^2 over 128 dims. */
+void oracle_host_lenght(const float* q, const float* r, const int64_t* qi, const int64_t* ri, int n,
+                        float* out);
 /* host_brute (src/knn_brute.cu:801-835) with host_lenght (:873-885). */
 void oracle_host_brute(const float* q, const float* r, int q_n, int r_n, float* sorted4);
 /* Same, queries split over `threads` OpenMP threads; identical results. */

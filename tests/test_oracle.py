@@ -310,3 +310,50 @@ def test_hash_loss_closed_form_and_gradients():
         fdz = (oracle.hash_loss(s, zp, B, nbits, al, be, ga) - oracle.hash_loss(s, zm, B, nbits, al, be, ga)) / (2 * eps)
         assert abs(dz[i, k] - dls[i, k] - fdz) < 1e-7, (i, k)  # straight-through + the gamma term
     assert (dz[:, nbits:] == 0).all() and (dls[:, nbits:] == 0).all()
+
+
+def numpy_seq_dist(q, r):
+    """Independent restatement of host_lenght for every (query, row): float32
+    numpy arrays, dims added one at a time (one rounding per subtraction,
+    product and addition; numpy never fuses them)."""
+    q, r = q.astype(np.float32), r.astype(np.float32)
+    d = np.zeros((q.shape[0], r.shape[0]), np.float32)
+    for i in range(128):
+        c = (q[:, i][:, None] - r[:, i][None, :]).astype(np.float32)
+        d = (d + (c * c).astype(np.float32)).astype(np.float32)
+    return d
+
+
+def test_host_lenght_is_the_sequential_float32_sum():
+    """oracle.host_lenght on float data equals the numpy sequential sum bit for
+    bit; a pairwise (tree) float32 sum of the same squares differs for some
+    pairs, so the check pins the summation order."""
+    q = datagen.unit(40, seed=31)
+    r = datagen.unit(60, seed=32) * np.float32(3.7)
+    want = numpy_seq_dist(q, r)
+    qi, ri = np.meshgrid(np.arange(40), np.arange(60), indexing="ij")
+    got = oracle.host_lenght(q, r, qi.ravel(), ri.ravel()).reshape(40, 60)
+    np.testing.assert_array_equal(got, want)
+    sq = ((q[:, None, :] - r[None, :, :]) ** 2).astype(np.float32)
+    tree = sq.sum(axis=2, dtype=np.float32)  # numpy's pairwise summation
+    assert (tree != want).any()
+
+
+def test_host_brute_on_float_data_matches_numpy():
+    """host_brute on unit-norm floats (the reference's own test data,
+    src/helper.cu:40-58) and on SIFT / 512 (src/test.cu:103) equals the
+    independent numpy top-2: sequential float32 distances, lexicographic
+    (distance, index) order, with near-duplicate queries (d << 1)."""
+    for r, seed in ((datagen.unit(257, seed=41), 42), (datagen.base(300, seed=43) / np.float32(512), 44)):
+        rng = np.random.default_rng(seed)
+        q = r[rng.integers(0, r.shape[0], 30)] + rng.normal(0, 1e-3, (30, 128)).astype(np.float32)
+        q = np.concatenate([q.astype(np.float32), datagen.unit(20, seed=seed)])
+        d = numpy_seq_dist(q, r)
+        s = oracle.host_brute(q, r)
+        for i in range(q.shape[0]):
+            order = np.lexsort((np.arange(r.shape[0]), d[i]))[:2]
+            assert s[i, 2] == order[0] and s[i, 3] == order[1]
+            assert s[i, 0] == d[i, order[0]] and s[i, 1] == d[i, order[1]]
+        assert (s[:30, 0] < 0.01).all()
+        got = oracle.host_lenght(q, r, np.arange(q.shape[0]), s[:, 2].astype(np.int64))
+        np.testing.assert_array_equal(got, s[:, 0])
