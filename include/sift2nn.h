@@ -202,6 +202,22 @@ int sift_bf_index_query(sift_handle_t handle, sift_bf_index_t index, const void*
                         uint32_t q_n, uint32_t* idx, float* dist, sift_float4* sorted,
                         uint32_t* matches, float ratio, int blocking);
 int sift_bf_index_destroy(sift_bf_index_t index);
+/* sift_bf_index_query without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Its scratch (packed queries, the
+ * fused kernel's partial top-2 and seed bounds) lives in one device array of
+ * SIFT_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; calls
+ * whose scratch would not fit (e.g. more than about 16k queries against 1M
+ * rows) return SIFT_ERR_INVALID.  Device query rows only (type 1..4); FP8
+ * indexes return SIFT_ERR_UNSUPPORTED.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other handles, other threads) race
+ * and must be serialised by the caller.  blocking = 0 enqueues on the
+ * handle's stream and returns (capturable: no memset, allocation or sync);
+ * input errors are then reported by the next blocking call. */
+#define SIFT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int sift_bf_index_query_noalloc(sift_handle_t handle, sift_bf_index_t index, const void* q, int type,
+                                uint32_t q_n, uint32_t* idx, float* dist, sift_float4* sorted,
+                                uint32_t* matches, float ratio, int blocking);
 
 /* LSH index.  Replaces the per-call table build of lsh_gpu
  * (src/lsh_h.cu:903-914) and lsh_thread_sort_reduce (include/lsh_h.h:66,

@@ -57,6 +57,7 @@ SIGNATURES = {
     "sift_bf_index_build": (i32, [vp, vp, i32, u32, i32, C.POINTER(vp)]),
     "sift_bf_index_query": (i32, [vp, vp, vp, i32, u32, vp, vp, vp, vp, f32, i32]),
     "sift_bf_index_destroy": (i32, [vp]),
+    "sift_bf_index_query_noalloc": (i32, [vp, vp, vp, i32, u32, vp, vp, vp, vp, f32, i32]),
     "sift_lsh_build": (i32, [vp, vp, i32, u32, i32, i32, i32, u64, C.POINTER(vp)]),
     "sift_lsh_query": (i32, [vp, vp, vp, i32, u32, vp, vp, vp, vp, f32, i32]),
     "sift_lsh_destroy": (i32, [vp]),

@@ -164,6 +164,7 @@ int sift_create(sift_handle_t* handle) {
         const long v = std::strtol(env, nullptr, 10);
         h->wait_polls = (int)std::max(0L, std::min(v, 1L << 24));
     }
+    (void)noalloc_init(h);  // on failure the no-allocation entry point reports it
     *handle = h;
     tl_status = SIFT_OK;
     return SIFT_OK;

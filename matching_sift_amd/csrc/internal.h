@@ -203,6 +203,12 @@ struct sift_bf_index {
     bool coarse = false;  // F32 over data that is not all integers in [0,255]
 };
 namespace sift {
+// No-allocation queries (noalloc.hip): the device scratch array's size, the
+// per-device address lookup done at handle creation, and the kernel that
+// sets the exactness flag where a memset node would not be capturable.
+constexpr size_t NOALLOC_SCRATCH_BYTES = SIFT_NOALLOC_SCRATCH_BYTES;
+int noalloc_init(sift_context* h);
+int set_flag_word(sift_context* h, int* p, int v);
 // Pack q_n queries into qset (work: sized in advance, no allocation) and run
 // the index's fused pass; flag = device int for the non-integer test.
 int bf_index_run(sift_context* h, sift_bf_index* ix, const void* q, int type, uint32_t q_n, const Knn2Out& out,

@@ -940,7 +940,7 @@ namespace sift {
 // F32 base) and the query rows as given when the keys are coarse.
 int bf_index_run(sift_context* h, sift_bf_index* ix, const void* q, int type, uint32_t q_n, const Knn2Out& out_in,
                  int* flag, PackedSet& qset, const Knn2Work* work) {
-    SIFT_HIP_TRY(h, hipMemsetAsync(flag, ix->coarse ? 1 : 0, 4, h->stream));
+    SIFT_TRY(set_flag_word(h, flag, ix->coarse ? 1 : 0));
     SIFT_TRY(work ? pack_launch(h, q, type, q_n, nullptr, ix->kind, PACK_QUERY, qset, flag)
                   : pack_side(h, q, type, q_n, query_rows_pad(ix->kind, q_n), ix->kind, PACK_QUERY, qset, flag));
     Knn2Out o = out_in;
