@@ -158,7 +158,8 @@ void cublas_2nn_brute_f(const des_t_f* q_points, const des_t_f* r_points, int q_
                         float* dist, sift_handle_t handle);
 
 /* Replaces int lsh_gpu(...) (include/lsh_h.h:69, src/lsh_h.cu:722): build an
- * LSH index over r with l tables of nbits bits, query it with q, write
+ * LSH index over r with l tables of nbits bits (the float mode when q or r is
+ * not all integers in [0,255]), query it with q, write
  * matches[i] = idx1 of the re-ranked candidates when the ratio test passes,
  * UINT32_MAX otherwise (also when a query has no candidate).  lsh_type selects
  * SIFT_LSH_HYPERPLANE (0); stream_n is accepted and unused.  Hyperplanes come
@@ -223,8 +224,14 @@ int sift_bf_index_query_noalloc(sift_handle_t handle, sift_bf_index_t index, con
  * (src/lsh_h.cu:903-914) and lsh_thread_sort_reduce (include/lsh_h.h:66,
  * src/lsh_h.cu:565): L tables of nbits-bit sign codes (bit b set when
  * h_b . x < 0), all tables' (code, row) pairs radix-sorted on the device.
- * Descriptors must hold integers in [0,255] (SIFT_ERR_NOT_EXACT otherwise).
- * nbits in [1,32], L in [1,256], nbits + ceil(log2 L) <= 32. */
+ * A base of integers in [0,255] hashes x itself and re-ranks candidates in
+ * exact int32 (queries must then be integers in [0,255] too:
+ * SIFT_ERR_NOT_EXACT otherwise).  Any other base (unit-norm floats, SIFT /
+ * 512, ...) builds the float mode: x is hashed as the integers
+ * clamp(rint(x * 2^k), -255, 255), k the largest with max|base| * 2^k <= 255,
+ * so codes stay exact, and candidates are re-ranked with the reference
+ * host_lenght's sequential FP32 distance over the rows as given; queries may
+ * be any data.  nbits in [1,32], L in [1,256], nbits + ceil(log2 L) <= 32. */
 typedef struct sift_lsh_index* sift_lsh_index_t;
 int sift_lsh_build(sift_handle_t handle, const void* r, int type, uint32_t r_n, int nbits, int L,
                    int hash_kind, uint64_t seed, sift_lsh_index_t* index);
@@ -239,6 +246,8 @@ int sift_lsh_destroy(sift_lsh_index_t index);
 int sift_lsh_codes(sift_handle_t handle, sift_lsh_index_t index, const void* q, int type,
                    uint32_t q_n, uint32_t* codes);
 int sift_lsh_info(sift_lsh_index_t index, uint32_t* r_n, int* nbits, int* L);
+/* 1 and the scale exponent k for a float-mode index, 0 and 0 otherwise. */
+int sift_lsh_hash_input(sift_lsh_index_t index, int* float_mode, int* scale_exp);
 /* Host copies: hyperplanes float[L][nbits][128]; sorted keys (t << nbits | code)
  * and base rows, uint32[L*r_n] each. */
 int sift_lsh_get_hyperplanes(sift_lsh_index_t index, float* planes);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "sift_lsh_destroy": (i32, [vp]),
     "sift_lsh_codes": (i32, [vp, vp, vp, i32, u32, vp]),
     "sift_lsh_info": (i32, [vp, C.POINTER(u32), C.POINTER(i32), C.POINTER(i32)]),
+    "sift_lsh_hash_input": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "sift_lsh_get_hyperplanes": (i32, [vp, vp]),
     "sift_lsh_get_buckets": (i32, [vp, vp, vp]),
     "sift_mlp_create": (i32, [vp, i32, u64, C.POINTER(vp)]),

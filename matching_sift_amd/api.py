@@ -237,6 +237,12 @@ class LSHIndex:
         self.ix = ix.value
         return self
 
+    def hash_input(self):
+        """(float_mode, k): a float-mode index hashes clamp(rint(x * 2^k), -255, 255)."""
+        fm, k = C.c_int(), C.c_int()
+        check(lib().sift_lsh_hash_input(self.ix, C.byref(fm), C.byref(k)), self.handle.h)
+        return bool(fm.value), k.value
+
     def set_probes(self, probes):
         """Multi-probe: also look up the buckets with each of the `probes`
         least confident bits flipped (0 = off)."""

@@ -94,7 +94,9 @@ struct PackedSet {
 // PACK_QUERY: float kinds store -2q (the fused kernel's operand).
 // PACK_PLAIN: values as given (hyperplanes, projection inputs).
 // `bad` (device int) is OR-ed with 1 if a value is not an integer in [0,255].
-enum PackRole { PACK_PLAIN = 0, PACK_BASE = 1, PACK_QUERY = 2 };
+// PACK_QUANT: plain BF16 tiles of clamp(rint(x * 2^qexp), -255, 255) (LSH
+//            hashing of data that is not all integers in [0,255]).
+enum PackRole { PACK_PLAIN = 0, PACK_BASE = 1, PACK_QUERY = 2, PACK_QUANT = 3 };
 int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int kind,
               int role, PackedSet& out, int* bad_dev);
 // Launch the pack kernel into an already sized set (o.rows_pad rows; no
@@ -102,6 +104,13 @@ int pack_rows(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
 // packed row -> input row, -1 for a pad row; without it rows >= n are pads.
 int pack_launch(sift_context* h, const void* in, int type, uint32_t n, const int32_t* rowmap, int kind,
                 int role, PackedSet& o, int* bad_dev);
+// PACK_QUANT tiles (KIND_BF16) of n rows with scale 2^qexp, qexp in [-100, 100].
+int pack_quant(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int qexp,
+               PackedSet& o, int* bad_dev);
+// Largest |value| of n rows as float bits, atomicMax-ed into *out_dev (caller zeroes it).
+int abs_max(sift_context* h, const void* in, int type, uint32_t n, uint32_t* out_dev);
+// float32 copy [n][128] of n input rows.
+int to_f32(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out);
 // uint8 copy [n][128] and int32 squared norms of n input rows (flags non-integers).
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,
           int* bad_dev);

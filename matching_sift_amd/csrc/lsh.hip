@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "fused.h"
+#include "rows.h"
 
 namespace sift {
 int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
@@ -53,7 +54,13 @@ struct sift_lsh_index {
     double cand_per_lookup = 0.0;  // mean bucket size seen by a base-like point (re-rank order rule)
     DevBuf keys, vals;          // sorted (table << b | code), base row index; L*n each
     DevBuf offs;                // b <= OFFS_MAX_BITS: bucket starts, [L][2^b + 1] absolute positions
-    DevBuf base_u8, base_nrm;   // gather copy
+    DevBuf base_u8, base_nrm;   // gather copy (integer data)
+    // Data that is not all integers in [0,255]: hashed as clamp(rint(x * 2^qexp),
+    // -255, 255) (integers, so codes stay exact) and re-ranked from a float32
+    // copy with the reference's sequential FP32 distance.
+    bool float_mode = false;
+    int qexp = 0;
+    DevBuf base_f32;
     DevBuf scratch_k, scratch_v, hist, levels[4];
 };
 
@@ -366,6 +373,66 @@ __global__ __launch_bounds__(256) void lsh_rerank(RerankArgs a) {
     if (lane == 0) write_result(a, q, m1, i1, m2, i2, nc);
 }
 
+// Float data: one wave per query, each lane one candidate row at a time (the
+// sequential FP32 distance of host_lenght over the float32 copy of the base
+// and the query row as given), top-2 per lane with rows already held skipped,
+// then a wave-wide merge that drops rows held twice.  Distances are >= +0, so
+// their bit patterns order as unsigned integers and the (distance, row) order
+// is the integer path's.
+struct RerankF32Args {
+    RerankArgs a;
+    const void* q;  // query rows of type q_type
+    int q_type;
+    const float* base;
+};
+__global__ __launch_bounds__(256) void lsh_rerank_f32(RerankF32Args f) {
+    const RerankArgs& a = f.a;
+    const int lane = threadIdx.x & 63;
+    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= a.q_n) return;  // whole wave
+    uint32_t m1 = 0xFFFFFFFFu, m2 = 0xFFFFFFFFu, i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
+    uint32_t nc = 0;
+    const int TP = a.L * (a.P + 1);
+    for (int tb = 0; tb < TP; tb += 64) {
+        uint32_t rlo = 0, rhi = 0;
+        if (tb + lane < TP) bucket_range(a, tb + lane, q, rlo, rhi);
+        const int nt = min(64, TP - tb);
+        for (int i = 0; i < nt; i++) {
+            const uint32_t lo = __shfl(rlo, i), hi = __shfl(rhi, i);
+            nc += hi - lo;
+            for (uint32_t p = lo + lane; p < hi; p += 64) {
+                const uint32_t j = a.vals[p];
+                if (j == i1 || j == i2) continue;
+                const uint32_t d = __float_as_uint(seq_dist(f.q, f.q_type, q, f.base, SIFT_FLOAT_DEVICE, j));
+                const bool c1 = lex_less(d, j, m1, i1), c2 = lex_less(d, j, m2, i2);
+                m2 = c1 ? m1 : (c2 ? d : m2);
+                i2 = c1 ? i1 : (c2 ? j : i2);
+                m1 = c1 ? d : m1;
+                i1 = c1 ? j : i1;
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t b1 = __shfl_xor(m1, off), ib1 = __shfl_xor(i1, off);
+        const uint32_t b2 = __shfl_xor(m2, off), ib2 = __shfl_xor(i2, off);
+        merge2_dedup(m1, i1, m2, i2, b1, ib1, b2, ib2);
+    }
+    if (lane != 0) return;
+    const bool h1 = i1 != 0xFFFFFFFFu, h2 = i2 != 0xFFFFFFFFu;
+    const float d1 = h1 ? __uint_as_float(m1) : 3.402823466e+38f, d2 = h2 ? __uint_as_float(m2) : 3.402823466e+38f;
+    if (a.idx) {
+        a.idx[2 * (size_t)q] = i1;
+        a.idx[2 * (size_t)q + 1] = i2;
+    }
+    if (a.dist) {
+        a.dist[2 * (size_t)q] = d1;
+        a.dist[2 * (size_t)q + 1] = d2;
+    }
+    if (a.ncand) a.ncand[q] = nc;
+    if (a.matches) a.matches[q] = (h1 && sqrtf(d1) / sqrtf(d2) < a.ratio) ? i1 : 0xFFFFFFFFu;
+}
+
 // Bucket-ordered variant: a unit is one (table, probe, query) lookup, u =
 // tp * q_n + q; units are visited in order of their bucket (sorted keys
 // (table << b) | code), so the waves that read one bucket's rows run side by
@@ -575,7 +642,7 @@ int rerank_units(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, cons
 void free_index(sift_lsh_index* ix) {
     if (ix->owns_mlps)
         for (sift_mlp* m : ix->mlps) sift_mlp_destroy(m);
-    for (DevBuf* b : {&ix->keys, &ix->vals, &ix->offs, &ix->base_u8, &ix->base_nrm, &ix->scratch_k,
+    for (DevBuf* b : {&ix->keys, &ix->vals, &ix->offs, &ix->base_u8, &ix->base_nrm, &ix->base_f32, &ix->scratch_k,
                       &ix->scratch_v, &ix->hist, &ix->hp.tiles, &ix->hp.norm, &ix->hp.norm_lane})
         release(*b);
     for (auto& l : ix->levels) release(l);
@@ -586,7 +653,10 @@ int check_flag(sift_context* h, int* flag, const char* what) {
     int bad = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
-    if (bad) return set_error(h, SIFT_ERR_NOT_EXACT, std::string(what) + ": LSH needs integer descriptors in [0,255]");
+    if (bad)
+        return set_error(h, SIFT_ERR_NOT_EXACT, std::string(what) +
+                                                    ": an index built from integer descriptors in [0,255] needs "
+                                                    "queries of that kind (build from float data for float queries)");
     return SIFT_OK;
 }
 
@@ -705,21 +775,66 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
 // Codes of every base row for all tables, stable radix sort of
 // (table << b | code, row), bucket starts, and the uint8 gather copy of the
 // base.
-int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag) {
+// Scale exponent of the hash input for data that is not all integers in
+// [0,255]: the largest k with max|x| * 2^k <= 255, within [-100, 100] (0 for
+// an all-zero base), so the hashed integers use the range of SIFT bytes.
+int hash_scale_exp(float mx) {
+    if (!(mx > 0.f) || !std::isfinite(mx)) return 0;
+    int k = (int)std::floor(std::log2(255.0 / (double)mx));
+    while (k < 100 && std::ldexp((double)mx, k + 1) <= 255.0) k++;
+    while (k > -100 && std::ldexp((double)mx, k) > 255.0) k--;
+    return std::max(-100, std::min(100, k));
+}
+
+// flag: the handle's LSH flag words ([0] exactness, [1] scratch, [3] max |x|).
+int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag, bool force_float) {
     const uint32_t r_n = ix->n;
     const int L = ix->L, nbits = ix->nbits;
     int tbits = 0;
     while ((1 << tbits) < L) tbits++;
     int rc = SIFT_OK;
     do {
-        // gather copy + exactness check
-        {
-            ProfScope ps(h, PROF_PACK);
-            if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
-        }
         const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-        if ((rc = pack_rows(h, r, type, r_n, rows_pad, KIND_BF16, PACK_PLAIN, h->b_set, flag)) != SIFT_OK) break;
-        if ((rc = check_flag(h, flag, "sift_lsh_build")) != SIFT_OK) break;
+        bool float_mode = force_float;
+        if (!float_mode) {  // integer data: uint8 gather copy, exactness check, plain tiles
+            {
+                ProfScope ps(h, PROF_PACK);
+                if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
+            }
+            if ((rc = pack_rows(h, r, type, r_n, rows_pad, KIND_BF16, PACK_PLAIN, h->b_set, flag)) != SIFT_OK) break;
+            int bad = 0;
+            if (hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                rc = set_error(h, SIFT_ERR_HIP, "build flag");
+                break;
+            }
+            float_mode = bad != 0;
+        }
+        if (float_mode) {  // quantised hash input, float32 gather copy
+            if (ix->hash_kind == SIFT_LSH_LEARNED) {
+                rc = set_error(h, SIFT_ERR_NOT_EXACT, "sift_lsh_build_learned: the learned hash needs integer descriptors in [0,255]");
+                break;
+            }
+            release(ix->base_u8);
+            release(ix->base_nrm);
+            uint32_t mx = 0;
+            auto* mx_dev = reinterpret_cast<uint32_t*>(flag + 3);
+            if (hipMemsetAsync(mx_dev, 0, 4, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
+            {
+                ProfScope ps(h, PROF_PACK);
+                if ((rc = abs_max(h, r, type, r_n, mx_dev)) != SIFT_OK) break;
+            }
+            if (hipMemcpyAsync(&mx, mx_dev, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                rc = set_error(h, SIFT_ERR_HIP, "build max");
+                break;
+            }
+            ix->float_mode = true;
+            ix->qexp = hash_scale_exp(__builtin_bit_cast(float, mx));
+            ProfScope ps(h, PROF_PACK);
+            if ((rc = to_f32(h, r, type, r_n, ix->base_f32)) != SIFT_OK) break;
+            if ((rc = pack_quant(h, r, type, r_n, rows_pad, ix->qexp, h->b_set, flag + 1)) != SIFT_OK) break;
+        }
         const size_t total = (size_t)r_n * L;
         for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
             if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
@@ -755,6 +870,15 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
     return rc;
 }
 
+// Query rows into sc->q_tiles the way the index hashes its base: plain BF16
+// (integer index) or the quantised input of a float index.
+int pack_for_hash(sift_context* h, const sift_lsh_index* ix, const void* q, int type, uint32_t q_n, LshScratch* sc,
+                  int* flag) {
+    const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
+    if (ix->float_mode) return pack_quant(h, q, type, q_n, rows_pad, ix->qexp, sc->q_tiles, flag + 1);
+    return pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, flag);
+}
+
 }  // namespace
 
 extern "C" {
@@ -772,8 +896,13 @@ void sift_lsh_state_free(sift_context* h) {
     h->lsh_state = nullptr;
 }
 
-int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int nbits, int L,
-                   int hash_kind, uint64_t seed, sift_lsh_index_t* out) {
+}  // extern "C"
+
+namespace {
+// force_float: build the float mode whatever the base holds (lsh_gpu, when the
+// queries are not all integers in [0,255]).
+int lsh_build_impl(sift_handle_t h, const void* r, int type, uint32_t r_n, int nbits, int L, int hash_kind,
+                   uint64_t seed, bool force_float, sift_lsh_index_t* out) {
     if (!h || !out) return set_error(h, SIFT_ERR_INVALID, "null handle/index");
     *out = nullptr;
     if (type < SIFT_FLOAT_HOST || type > SIFT_U8_DEVICE) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
@@ -786,7 +915,10 @@ int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int n
         return set_error(h, SIFT_ERR_UNSUPPORTED, "hash kind not supported by sift_lsh_build");
     if ((uint64_t)r_n * (uint64_t)L >= (1ull << 32)) return set_error(h, SIFT_ERR_UNSUPPORTED, "r_n * L too large");
     if (r_n > 0 && !r) return set_error(h, SIFT_ERR_INVALID, "null base");
-    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(h, r, r_n, h->stage_r, &r));
+        type = SIFT_FLOAT_DEVICE;
+    }
 
     auto* ix = new sift_lsh_index();
     ix->h = h;
@@ -803,7 +935,7 @@ int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int n
         int* flag = static_cast<int*>(sc->flag.p);
         if (hipMemsetAsync(flag, 0, 8, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
         if ((rc = pack_planes(h, ix, flag + 1)) != SIFT_OK) break;
-        rc = build_tables(h, ix, r, type, flag);
+        rc = build_tables(h, ix, r, type, flag, force_float);
     } while (0);
     if (rc != SIFT_OK) {
         free_index(ix);
@@ -811,6 +943,14 @@ int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int n
     }
     *out = ix;
     return SIFT_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sift_lsh_build(sift_handle_t h, const void* r, int type, uint32_t r_n, int nbits, int L,
+                   int hash_kind, uint64_t seed, sift_lsh_index_t* out) {
+    return lsh_build_impl(h, r, type, r_n, nbits, L, hash_kind, seed, false, out);
 }
 
 int sift_lsh_codes(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type, uint32_t q_n,
@@ -821,8 +961,7 @@ int sift_lsh_codes(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     LshScratch* sc = scratch_of(h);
     SIFT_TRY(reserve(h, sc->flag, 16));
     int* flag = static_cast<int*>(sc->flag.p);
-    const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
-    SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, flag));
+    SIFT_TRY(pack_for_hash(h, ix, q, type, q_n, sc, flag));
     SIFT_TRY(hash_codes(h, ix, sc->q_tiles, q_n, codes, q_n, false));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     return SIFT_OK;
@@ -835,14 +974,16 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     if (type < SIFT_FLOAT_HOST || type > SIFT_U8_DEVICE) return set_error(h, SIFT_ERR_INVALID, "unknown input type");
     if (q_n == 0) return SIFT_OK;
     if (!q) return set_error(h, SIFT_ERR_INVALID, "null queries");
-    if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
+        type = SIFT_FLOAT_DEVICE;
+    }
     LshScratch* sc = scratch_of(h);
     SIFT_TRY(reserve(h, sc->flag, 16));
     int* flag = static_cast<int*>(sc->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 8, h->stream));
-    SIFT_TRY(to_u8(h, q, type, q_n, sc->q_u8, sc->q_nrm, flag));
-    const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
-    SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, flag));
+    if (!ix->float_mode) SIFT_TRY(to_u8(h, q, type, q_n, sc->q_u8, sc->q_nrm, flag));
+    SIFT_TRY(pack_for_hash(h, ix, q, type, q_n, sc, flag));
     const uint32_t* qcodes = nullptr;
     {
         ProfScope ps(h, PROF_LSH_PROJ);
@@ -866,6 +1007,14 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     a.ncand = ncand;
     a.matches = matches;
     a.ratio = ratio;
+    if (ix->float_mode) {  // any query data; sequential FP32 distances from the float copy
+        ProfScope ps(h, PROF_LSH_QUERY);
+        RerankF32Args f{a, q, type, static_cast<const float*>(ix->base_f32.p)};
+        lsh_rerank_f32<<<(q_n + 3) / 4, 256, 0, h->stream>>>(f);
+        SIFT_HIP_TRY(h, hipGetLastError());
+        if (blocking) SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+        return SIFT_OK;
+    }
     {
         ProfScope ps(h, PROF_LSH_QUERY);
         if (bucket_ordered(ix, q_n)) {
@@ -902,8 +1051,7 @@ int sift_lsh_probe_codes(sift_handle_t h, sift_lsh_index_t ix, const void* q, in
     if (type == SIFT_FLOAT_HOST) SIFT_TRY(stage_host_input(h, q, q_n, h->stage_q, &q));
     LshScratch* sc = scratch_of(h);
     SIFT_TRY(reserve(h, sc->flag, 16));
-    const uint32_t rows_pad = ((q_n + 31) / 32) * 32;
-    SIFT_TRY(pack_rows(h, q, type, q_n, rows_pad, KIND_BF16, PACK_PLAIN, sc->q_tiles, static_cast<int*>(sc->flag.p)));
+    SIFT_TRY(pack_for_hash(h, ix, q, type, q_n, sc, static_cast<int*>(sc->flag.p)));
     const uint32_t* pc = nullptr;
     SIFT_TRY(query_codes(h, ix, sc, q_n, &pc));
     SIFT_HIP_TRY(h, hipMemcpyAsync(codes, pc, (size_t)ix->L * (ix->probes + 1) * q_n * 4, hipMemcpyDeviceToDevice,
@@ -942,7 +1090,7 @@ int sift_lsh_build_learned(sift_handle_t h, const void* r, int type, uint32_t r_
         if ((rc = reserve(h, sc->flag, 16)) != SIFT_OK) break;
         int* flag = static_cast<int*>(sc->flag.p);
         if (hipMemsetAsync(flag, 0, 8, h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "memset"); break; }
-        rc = build_tables(h, ix, r, type, flag);
+        rc = build_tables(h, ix, r, type, flag, false);
     } while (0);
     if (rc != SIFT_OK) {
         free_index(ix);
@@ -993,7 +1141,7 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     // the LSH pass's own input check (the non-blocking query skipped it)
-    SIFT_TRY(check_flag(h, static_cast<int*>(sc->flag.p), "sift_lsh_query_fallback"));
+    if (!ix->float_mode) SIFT_TRY(check_flag(h, static_cast<int*>(sc->flag.p), "sift_lsh_query_fallback"));
     if (n_fallback) *n_fallback = cnt;
     if (cnt == 0) return SIFT_OK;
     const uint32_t rows_pad = query_rows_pad(bf->kind, cnt);
@@ -1018,14 +1166,23 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     auto* od = reinterpret_cast<float*>(oi + 2 * (size_t)cnt);
     auto* om = oi + 4 * (size_t)cnt;
     Knn2Out o{oi, od, nullptr, om, ratio};
-    if (bf->kind == KIND_F32) {  // an F32 index over other data: exact re-rank from its tiles
+    if (bf->kind == KIND_BF16 || bf->kind == KIND_F16 || bf->kind == KIND_F32) {
+        // coarse keys for data that is not all integers in [0,255]: exact re-rank from the index's tiles
         o.coarse = flag;
-        o.src = ExactSrc{q, type, bf->base.tiles.p, SRC_F32_TILES, fb_map};
+        o.src = ExactSrc{q, type, bf->base.tiles.p,
+                         bf->kind == KIND_F32 ? SRC_F32_TILES : (bf->kind == KIND_BF16 ? SRC_BF16_TILES : SRC_F16_TILES),
+                         fb_map};
     }
     SIFT_TRY(knn2_packed(h, qs, bf->base, cnt, bf->r_n, o));
     scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(fb_map, cnt, oi, od, om, idx, dist, matches);
     SIFT_HIP_TRY(h, hipGetLastError());
-    return check_wait(h, "sift_lsh_query_fallback");
+    SIFT_TRY(check_wait(h, "sift_lsh_query_fallback"));
+    if (bf->kind == KIND_I8 || bf->kind == KIND_FP8) {  // those kinds cannot hold other data
+        int bad = 0;
+        SIFT_HIP_TRY(h, hipMemcpy(&bad, flag, 4, hipMemcpyDeviceToHost));
+        if (bad) return set_error(h, SIFT_ERR_NOT_EXACT, "sift_lsh_query_fallback: an I8 or FP8 brute-force index needs integer queries in [0,255]");
+    }
+    return SIFT_OK;
 }
 
 // ------------------------------------------------------- persistence ----
@@ -1044,10 +1201,14 @@ struct LshFileHeader {
 };
 constexpr char LSH_MAGIC[8] = {'S', 'I', 'F', 'T', 'L', 'S', 'H', '1'};
 
+// Version 1: integer index (uint8 rows + int32 norms).  Version 2: float
+// index (int32 scale exponent + 4 bytes reserved after the header, float32
+// rows).
 size_t lsh_file_bytes(const LshFileHeader& hd) {
     const size_t hash = hd.hash_kind == SIFT_LSH_LEARNED ? (size_t)hd.L * hd.nparam * 4
                                                           : (size_t)hd.L * hd.nbits * 128 * 4;
-    return sizeof(LshFileHeader) + hash + (size_t)hd.L * hd.n * 8 + (size_t)hd.n * 132;
+    const size_t rows = hd.version == 2 ? 8 + (size_t)hd.n * 512 : (size_t)hd.n * 132;
+    return sizeof(LshFileHeader) + hash + (size_t)hd.L * hd.n * 8 + rows;
 }
 }  // namespace
 
@@ -1057,7 +1218,7 @@ int sift_lsh_save(sift_lsh_index_t ix, const char* path) {
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
     LshFileHeader hd{};
     std::memcpy(hd.magic, LSH_MAGIC, 8);
-    hd.version = 1;
+    hd.version = ix->float_mode ? 2 : 1;
     hd.n = ix->n;
     hd.nbits = ix->nbits;
     hd.L = ix->L;
@@ -1073,23 +1234,30 @@ int sift_lsh_save(sift_lsh_index_t ix, const char* path) {
     }
     const size_t nk = (size_t)ix->L * ix->n;
     std::vector<uint32_t> keys(nk), vals(nk);
-    std::vector<uint8_t> u8((size_t)ix->n * 128);
-    std::vector<int32_t> nrm(ix->n);
+    const bool fm = ix->float_mode;
+    std::vector<uint8_t> u8(fm ? 0 : (size_t)ix->n * 128);
+    std::vector<int32_t> nrm(fm ? 0 : ix->n);
+    std::vector<float> f32(fm ? (size_t)ix->n * 128 : 0);
     if (nk) {
         SIFT_HIP_TRY(h, hipMemcpy(keys.data(), ix->keys.p, nk * 4, hipMemcpyDeviceToHost));
         SIFT_HIP_TRY(h, hipMemcpy(vals.data(), ix->vals.p, nk * 4, hipMemcpyDeviceToHost));
     }
-    if (ix->n) {
+    if (ix->n && fm) {
+        SIFT_HIP_TRY(h, hipMemcpy(f32.data(), ix->base_f32.p, f32.size() * 4, hipMemcpyDeviceToHost));
+    } else if (ix->n) {
         SIFT_HIP_TRY(h, hipMemcpy(u8.data(), ix->base_u8.p, u8.size(), hipMemcpyDeviceToHost));
         SIFT_HIP_TRY(h, hipMemcpy(nrm.data(), ix->base_nrm.p, nrm.size() * 4, hipMemcpyDeviceToHost));
     }
     FILE* f = std::fopen(path, "wb");
     if (!f) return set_error(h, SIFT_ERR_INVALID, std::string("cannot write ") + path);
     bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1;
+    const int32_t ext[2] = {ix->qexp, 0};
+    ok = ok && (!fm || std::fwrite(ext, 4, 2, f) == 2);
     ok = ok && (hash.empty() || std::fwrite(hash.data(), 4, hash.size(), f) == hash.size());
     ok = ok && (nk == 0 || (std::fwrite(keys.data(), 4, nk, f) == nk && std::fwrite(vals.data(), 4, nk, f) == nk));
-    ok = ok && (ix->n == 0 || (std::fwrite(u8.data(), 1, u8.size(), f) == u8.size() &&
-                               std::fwrite(nrm.data(), 4, nrm.size(), f) == nrm.size()));
+    ok = ok && (ix->n == 0 || fm || (std::fwrite(u8.data(), 1, u8.size(), f) == u8.size() &&
+                                     std::fwrite(nrm.data(), 4, nrm.size(), f) == nrm.size()));
+    ok = ok && (ix->n == 0 || !fm || std::fwrite(f32.data(), 4, f32.size(), f) == f32.size());
     ok = (std::fclose(f) == 0) && ok;
     return ok ? SIFT_OK : set_error(h, SIFT_ERR_INVALID, std::string("write failed: ") + path);
 }
@@ -1104,6 +1272,8 @@ struct LshFile {
     std::vector<uint32_t> keys, vals;
     std::vector<uint8_t> u8;
     std::vector<int32_t> nrm;
+    int32_t qexp = 0;        // version 2
+    std::vector<float> f32;  // version 2
 };
 
 // Read and check an index file without touching the device.  The contents
@@ -1116,31 +1286,41 @@ std::string read_lsh_file(const char* path, LshFile& lf) {
     if (!f) return "cannot open " + p;
     struct Closer { FILE* f; ~Closer() { std::fclose(f); } } closer{f};
     LshFileHeader& hd = lf.hd;
-    if (std::fread(&hd, sizeof hd, 1, f) != 1 || std::memcmp(hd.magic, LSH_MAGIC, 8) != 0 || hd.version != 1)
+    if (std::fread(&hd, sizeof hd, 1, f) != 1 || std::memcmp(hd.magic, LSH_MAGIC, 8) != 0 ||
+        (hd.version != 1 && hd.version != 2))
         return p + " is not an LSH index file";
+    const bool fm = hd.version == 2;
     int tbits = 0;
     while (hd.L > 0 && (1 << tbits) < hd.L) tbits++;
     const bool learned = hd.hash_kind == SIFT_LSH_LEARNED;
     if (hd.nbits < 1 || hd.nbits > 32 || hd.L < 1 || hd.L > 256 || hd.nbits + tbits > 32 ||
         (uint64_t)hd.n * (uint64_t)hd.L >= (1ull << 32) || hd.n >= (1u << 24) ||
         (hd.hash_kind != SIFT_LSH_HYPERPLANE && hd.hash_kind != SIFT_LSH_HYPERPLANE_PM1 && !learned) ||
-        (learned && hd.nparam != (uint32_t)SIFT_MLP_NPARAM) || (!learned && hd.nparam != 0))
+        (learned && hd.nparam != (uint32_t)SIFT_MLP_NPARAM) || (!learned && hd.nparam != 0) || (fm && learned))
         return p + ": bad index header";
     std::fseek(f, 0, SEEK_END);
     const long size = std::ftell(f);
     if (size < 0 || (size_t)size != lsh_file_bytes(hd)) return p + ": size does not match its header";
     std::fseek(f, (long)sizeof hd, SEEK_SET);
+    int32_t ext[2] = {0, 0};
+    if (fm && std::fread(ext, 4, 2, f) != 2) return "short read: " + p;
+    lf.qexp = ext[0];
+    if (fm && (lf.qexp < -100 || lf.qexp > 100)) return p + ": bad scale exponent";
     lf.hash.resize(learned ? (size_t)hd.L * hd.nparam : (size_t)hd.L * hd.nbits * 128);
     const size_t nk = (size_t)hd.L * hd.n;
     lf.keys.resize(nk);
     lf.vals.resize(nk);
-    lf.u8.resize((size_t)hd.n * 128);
-    lf.nrm.resize(hd.n);
+    lf.u8.resize(fm ? 0 : (size_t)hd.n * 128);
+    lf.nrm.resize(fm ? 0 : hd.n);
+    lf.f32.resize(fm ? (size_t)hd.n * 128 : 0);
     bool ok = std::fread(lf.hash.data(), 4, lf.hash.size(), f) == lf.hash.size();
     ok = ok && (nk == 0 || (std::fread(lf.keys.data(), 4, nk, f) == nk && std::fread(lf.vals.data(), 4, nk, f) == nk));
-    ok = ok && (hd.n == 0 || (std::fread(lf.u8.data(), 1, lf.u8.size(), f) == lf.u8.size() &&
-                              std::fread(lf.nrm.data(), 4, lf.nrm.size(), f) == lf.nrm.size()));
+    ok = ok && (hd.n == 0 || fm || (std::fread(lf.u8.data(), 1, lf.u8.size(), f) == lf.u8.size() &&
+                                    std::fread(lf.nrm.data(), 4, lf.nrm.size(), f) == lf.nrm.size()));
+    ok = ok && (hd.n == 0 || !fm || std::fread(lf.f32.data(), 4, lf.f32.size(), f) == lf.f32.size());
     if (!ok) return "short read: " + p;
+    for (float v : lf.f32)
+        if (!std::isfinite(v)) return p + ": a stored row holds a value that is not finite";
     for (size_t t = 0; t < (size_t)hd.L; t++) {
         const uint32_t prefix = hd.nbits < 32 ? (uint32_t)t : 0u;
         for (size_t i = t * hd.n; i < (t + 1) * hd.n; i++) {
@@ -1149,7 +1329,7 @@ std::string read_lsh_file(const char* path, LshFile& lf) {
             if (i > t * hd.n && lf.keys[i] < lf.keys[i - 1]) return p + ": the keys of a table are not sorted";
         }
     }
-    for (size_t j = 0; j < hd.n; j++) {
+    for (size_t j = 0; j < (fm ? 0 : hd.n); j++) {
         int32_t s = 0;
         for (int d = 0; d < 128; d++) s += (int32_t)lf.u8[j * 128 + d] * lf.u8[j * 128 + d];
         if (s != lf.nrm[j]) return p + ": a stored norm does not match its row";
@@ -1214,12 +1394,21 @@ int sift_lsh_load(sift_handle_t h, const char* path, sift_lsh_index_t* out) {
         for (DevBuf* b : {&ix->keys, &ix->vals})
             if ((rc = reserve(h, *b, std::max<size_t>(nk, 1) * 4)) != SIFT_OK) break;
         if (rc != SIFT_OK) break;
-        if ((rc = reserve(h, ix->base_u8, std::max<size_t>(u8.size(), 1))) != SIFT_OK) break;
-        if ((rc = reserve(h, ix->base_nrm, std::max<size_t>(nrm.size(), 1) * 4)) != SIFT_OK) break;
+        ix->float_mode = hd.version == 2;
+        ix->qexp = lf.qexp;
+        if (ix->float_mode) {
+            if ((rc = reserve(h, ix->base_f32, std::max<size_t>(lf.f32.size(), 1) * 4)) != SIFT_OK) break;
+        } else {
+            if ((rc = reserve(h, ix->base_u8, std::max<size_t>(u8.size(), 1))) != SIFT_OK) break;
+            if ((rc = reserve(h, ix->base_nrm, std::max<size_t>(nrm.size(), 1) * 4)) != SIFT_OK) break;
+        }
         if ((nk && (hipMemcpy(ix->keys.p, keys.data(), nk * 4, hipMemcpyHostToDevice) != hipSuccess ||
                     hipMemcpy(ix->vals.p, vals.data(), nk * 4, hipMemcpyHostToDevice) != hipSuccess)) ||
-            (hd.n && (hipMemcpy(ix->base_u8.p, u8.data(), u8.size(), hipMemcpyHostToDevice) != hipSuccess ||
-                      hipMemcpy(ix->base_nrm.p, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice) != hipSuccess))) {
+            (hd.n && !ix->float_mode &&
+             (hipMemcpy(ix->base_u8.p, u8.data(), u8.size(), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(ix->base_nrm.p, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) ||
+            (hd.n && ix->float_mode &&
+             hipMemcpy(ix->base_f32.p, lf.f32.data(), lf.f32.size() * 4, hipMemcpyHostToDevice) != hipSuccess)) {
             rc = set_error(h, SIFT_ERR_HIP, "copy index to the device");
             break;
         }
@@ -1238,6 +1427,13 @@ int sift_lsh_destroy(sift_lsh_index_t ix) {
     if (!ix) return SIFT_OK;
     (void)hipStreamSynchronize(ix->h->stream);
     free_index(ix);
+    return SIFT_OK;
+}
+
+int sift_lsh_hash_input(sift_lsh_index_t ix, int* float_mode, int* scale_exp) {
+    if (!ix) return set_error(nullptr, SIFT_ERR_INVALID, "null index");
+    if (float_mode) *float_mode = ix->float_mode ? 1 : 0;
+    if (scale_exp) *scale_exp = ix->qexp;
     return SIFT_OK;
 }
 
@@ -1274,8 +1470,28 @@ int lsh_gpu(void* q_points, void* r_points, int type, uint32_t q_n, uint32_t r_n
     if (lsh_type != SIFT_LSH_HYPERPLANE && lsh_type != SIFT_LSH_HYPERPLANE_PM1)
         return set_error(handle, SIFT_ERR_UNSUPPORTED, "lsh_gpu: lsh_type must be a hyperplane kind");
     if (!matches && q_n) return set_error(handle, SIFT_ERR_INVALID, "lsh_gpu: null matches");
+    if (!valid_type(type)) return set_error(handle, SIFT_ERR_INVALID, "lsh_gpu: unknown input type");
+    if ((q_n && !q_points) || (r_n && !r_points)) return set_error(handle, SIFT_ERR_INVALID, "lsh_gpu: null input");
+    const void* q = q_points;
+    const void* r = r_points;
+    if (type == SIFT_FLOAT_HOST) {
+        SIFT_TRY(stage_host_input(handle, q, q_n, handle->stage_q, &q));
+        SIFT_TRY(stage_host_input(handle, r, r_n, handle->stage_r, &r));
+        type = SIFT_FLOAT_DEVICE;
+    }
+    // queries that are not all integers in [0,255] need the float mode of the
+    // index whatever the base holds (the reference converts both to fp16,
+    // src/lsh_h.cu:748-780)
+    LshScratch* sc = scratch_of(handle);
+    SIFT_TRY(reserve(handle, sc->flag, 16));
+    int* flag = static_cast<int*>(sc->flag.p);
+    SIFT_HIP_TRY(handle, hipMemsetAsync(flag, 0, 4, handle->stream));
+    SIFT_TRY(check_exact(handle, q, type, q_n, flag));
+    int q_bad = 0;
+    SIFT_HIP_TRY(handle, hipMemcpyAsync(&q_bad, flag, 4, hipMemcpyDeviceToHost, handle->stream));
+    SIFT_HIP_TRY(handle, hipStreamSynchronize(handle->stream));
     sift_lsh_index_t ix = nullptr;
-    SIFT_TRY(sift_lsh_build(handle, r_points, type, r_n, nbits, l, lsh_type, handle->seed, &ix));
+    SIFT_TRY(lsh_build_impl(handle, r, type, r_n, nbits, l, lsh_type, handle->seed, q_bad != 0, &ix));
     DevBuf tmp;
     uint32_t* dst = matches;
     int rc = SIFT_OK;
@@ -1284,7 +1500,7 @@ int lsh_gpu(void* q_points, void* r_points, int type, uint32_t q_n, uint32_t r_n
         dst = static_cast<uint32_t*>(tmp.p);
     }
     if (rc == SIFT_OK)
-        rc = sift_lsh_query(handle, ix, q_points, type, q_n, nullptr, nullptr, nullptr, dst, threshold, 1);
+        rc = sift_lsh_query(handle, ix, q, type, q_n, nullptr, nullptr, nullptr, dst, threshold, 1);
     if (rc == SIFT_OK && dst != matches && q_n &&
         hipMemcpy(matches, dst, (size_t)q_n * 4, hipMemcpyDeviceToHost) != hipSuccess)
         rc = set_error(handle, SIFT_ERR_HIP, "lsh_gpu: copy matches to host");

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
                                                    float* __restrict__ norm_lane,
                                                    int* __restrict__ norm_half, int role,
                                                    int* __restrict__ bad,
-                                                   const int32_t* __restrict__ rowmap) {
+                                                   const int32_t* __restrict__ rowmap, int qexp) {
     const bool is_base = role == PACK_BASE;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t row = gid >> 4;
@@ -120,6 +120,11 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
 #pragma unroll
         for (int i = 0; i < 8; i++) ok &= exact_u8(v[i]);
         flag_wave(!ok, bad);
+        if (role == PACK_QUANT) {  // integers clamp(rint(x * 2^qexp), -255, 255): exact in BF16
+            const float sc = __uint_as_float((uint32_t)(127 + qexp) << 23);
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = fminf(fmaxf(rintf(v[i] * sc), -255.f), 255.f);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < 8; i++) v[i] = (KIND == KIND_I8) ? 128.f : 0.f;
@@ -365,9 +370,42 @@ __global__ __launch_bounds__(256) void to_u8_kernel(const void* __restrict__ in,
     if (blk == 0) nrm[row] = s;
 }
 
+// Largest |value| of n rows (float bit patterns of non-negative values order
+// as unsigned integers): one atomicMax per wave.
+template <int TYPE>
+__global__ __launch_bounds__(256) void absmax_kernel(const void* __restrict__ in, uint32_t n,
+                                                     uint32_t* __restrict__ out) {
+    const size_t total = (size_t)n * 16;
+    float m = 0.f;
+    for (size_t gid = blockIdx.x * (size_t)blockDim.x + threadIdx.x; gid < total;
+         gid += (size_t)gridDim.x * blockDim.x) {
+        float v[8];
+        load8<TYPE>(in, gid >> 4, (int)(gid & 15), v);
+#pragma unroll
+        for (int i = 0; i < 8; i++) m = fmaxf(m, fabsf(v[i]));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(out, __float_as_uint(m));
+}
+
+// float32 copy [n][128] of n input rows (exact conversions).
+template <int TYPE>
+__global__ __launch_bounds__(256) void to_f32_kernel(const void* __restrict__ in, uint32_t n,
+                                                     float4* __restrict__ out) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row = gid >> 4;
+    const int blk = gid & 15;
+    if (row >= n) return;
+    float v[8];
+    load8<TYPE>(in, row, blk, v);
+    out[((size_t)row * 16 + blk) * 2] = make_float4(v[0], v[1], v[2], v[3]);
+    out[((size_t)row * 16 + blk) * 2 + 1] = make_float4(v[4], v[5], v[6], v[7]);
+}
+
 template <int TYPE>
 int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad, int kind,
-                  int role, PackedSet& o, int* bad, const int32_t* rowmap) {
+                  int role, PackedSet& o, int* bad, const int32_t* rowmap, int qexp = 0) {
     const size_t threads = (size_t)rows_pad * 16;
     const unsigned grid = (unsigned)((threads + 255) / 256);
     if (grid == 0) return SIFT_OK;
@@ -377,19 +415,19 @@ int launch_pack_t(sift_context* h, const void* in, uint32_t n, uint32_t rows_pad
     auto* nh = static_cast<int*>(o.norm_half.p);
     switch (kind) {
         case KIND_BF16:
-            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_BF16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap, qexp);
             break;
         case KIND_F16:
-            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_F16><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap, qexp);
             break;
         case KIND_F32:
-            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_F32><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap, qexp);
             break;
         case KIND_I8:
-            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_I8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap, qexp);
             break;
         case KIND_FP8:
-            pack_kernel<TYPE, KIND_FP8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap);
+            pack_kernel<TYPE, KIND_FP8><<<grid, 256, 0, h->stream>>>(in, n, rows_pad, tiles, norm, nl, nh, role, bad, rowmap, qexp);
             break;
         default: return set_error(h, SIFT_ERR_INVALID, "pack: bad kind");
     }
@@ -458,6 +496,64 @@ int pack_side(sift_context* h, const void* in, int type, uint32_t n, uint32_t ro
 
 void release_set(PackedSet& s) {
     for (DevBuf* b : {&s.tiles, &s.norm, &s.norm_lane, &s.norm_half, &s.u8, &s.n2}) release(*b);
+}
+
+int pack_quant(sift_context* h, const void* in, int type, uint32_t n, uint32_t rows_pad, int qexp,
+               PackedSet& o, int* bad_dev) {
+    if (rows_pad % TILE_ROWS) return set_error(h, SIFT_ERR_INVALID, "pack: rows_pad not a multiple of 32");
+    if (qexp < -100 || qexp > 100) return set_error(h, SIFT_ERR_INVALID, "pack: scale exponent out of range");
+    const uint32_t ntiles = rows_pad / TILE_ROWS;
+    SIFT_TRY(reserve(h, o.tiles, (size_t)ntiles * tile_bytes(KIND_BF16)));
+    SIFT_TRY(reserve(h, o.norm, (size_t)rows_pad * 4));
+    o.rows = n;
+    o.rows_pad = rows_pad;
+    o.ntiles = ntiles;
+    o.kind = KIND_BF16;
+    ProfScope ps(h, PROF_PACK);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE:
+            return launch_pack_t<SIFT_FLOAT_DEVICE>(h, in, n, rows_pad, KIND_BF16, PACK_QUANT, o, bad_dev, nullptr, qexp);
+        case SIFT_HALF_DEVICE:
+            return launch_pack_t<SIFT_HALF_DEVICE>(h, in, n, rows_pad, KIND_BF16, PACK_QUANT, o, bad_dev, nullptr, qexp);
+        case SIFT_BF16_DEVICE:
+            return launch_pack_t<SIFT_BF16_DEVICE>(h, in, n, rows_pad, KIND_BF16, PACK_QUANT, o, bad_dev, nullptr, qexp);
+        case SIFT_U8_DEVICE:
+            return launch_pack_t<SIFT_U8_DEVICE>(h, in, n, rows_pad, KIND_BF16, PACK_QUANT, o, bad_dev, nullptr, qexp);
+        default: return set_error(h, SIFT_ERR_INVALID, "pack: unknown input type " + std::to_string(type));
+    }
+}
+
+int abs_max(sift_context* h, const void* in, int type, uint32_t n, uint32_t* out_dev) {
+    if (n == 0) return SIFT_OK;
+    const unsigned grid = (unsigned)std::min<size_t>(((size_t)n * 16 + 255) / 256, 4096);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE: absmax_kernel<SIFT_FLOAT_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, out_dev); break;
+        case SIFT_HALF_DEVICE: absmax_kernel<SIFT_HALF_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, out_dev); break;
+        case SIFT_BF16_DEVICE: absmax_kernel<SIFT_BF16_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, out_dev); break;
+        case SIFT_U8_DEVICE: absmax_kernel<SIFT_U8_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, out_dev); break;
+        default: return set_error(h, SIFT_ERR_INVALID, "abs_max: unknown input type");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
+int to_f32(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out) {
+    SIFT_TRY(reserve(h, out, std::max<size_t>((size_t)n * 512, 1)));
+    if (n == 0) return SIFT_OK;
+    const unsigned grid = (unsigned)(((size_t)n * 16 + 255) / 256);
+    auto* o = static_cast<float4*>(out.p);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE: to_f32_kernel<SIFT_FLOAT_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o); break;
+        case SIFT_HALF_DEVICE: to_f32_kernel<SIFT_HALF_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o); break;
+        case SIFT_BF16_DEVICE: to_f32_kernel<SIFT_BF16_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o); break;
+        case SIFT_U8_DEVICE: to_f32_kernel<SIFT_U8_DEVICE><<<grid, 256, 0, h->stream>>>(in, n, o); break;
+        default: return set_error(h, SIFT_ERR_INVALID, "to_f32: unknown input type");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
 }
 
 int to_u8(sift_context* h, const void* in, int type, uint32_t n, DevBuf& out, DevBuf& nrm,

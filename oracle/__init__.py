@@ -12,6 +12,9 @@ Function -> reference:
   lsh_codes         src/lsh_h.cu:265-317 (bit b set when h_b.x < 0)
   lsh_sort          src/lsh_h.cu:587-591 (stable by (code, index))
   lsh_query         src/lsh_h.cu:598-620, :657-681 (same-bucket candidates)
+  lsh_quantize,     float mode of the index (data not all integers in [0,255]):
+  lsh_scale_exp,    the integers hashed for x, and the re-rank by host_lenght
+  lsh_query_f32
   mlp_forward       learned hash (no reference counterpart)
 """
 import ctypes as C
@@ -55,6 +58,12 @@ def _lib():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = None
+        L.oracle_lsh_quantize.argtypes = [vp, C.c_size_t, i32, vp]
+        L.oracle_lsh_quantize.restype = None
+        L.oracle_lsh_scale_exp.argtypes = [vp, C.c_size_t]
+        L.oracle_lsh_scale_exp.restype = i32
+        L.oracle_lsh_query_f32.argtypes = [vp, vp, i32, i32, i32, vp, vp, vp, vp, vp, i32]
+        L.oracle_lsh_query_f32.restype = None
         L.oracle_hash_loss.argtypes = [vp, vp, i32, i32, C.c_double, C.c_double, C.c_double]
         L.oracle_hash_loss.restype = C.c_double
         L.oracle_hash_loss_grad.argtypes = [vp, i32, i32, C.c_double, C.c_double, C.c_double, vp, vp]
@@ -150,6 +159,32 @@ def lsh_query(q, r, codes_q, codes_r, threads=8):
     _lib().oracle_lsh_query(q.ctypes.data, r.ctypes.data, q.shape[0], r.shape[0], L,
                             cq.ctypes.data, cr.ctypes.data, idx.ctypes.data, dist.ctypes.data,
                             nc.ctypes.data, threads)
+    return idx, dist, nc
+
+
+def lsh_quantize(x, k):
+    """Float-mode hash input: clamp(rint(x * 2^k), -255, 255)."""
+    x = _f32(x)
+    out = np.empty(x.shape, np.float32)
+    _lib().oracle_lsh_quantize(x.ctypes.data, x.size, int(k), out.ctypes.data)
+    return out
+
+
+def lsh_scale_exp(x):
+    x = _f32(x)
+    return int(_lib().oracle_lsh_scale_exp(x.ctypes.data, x.size))
+
+
+def lsh_query_f32(q, r, codes_q, codes_r, threads=8):
+    """Float mode: codes_q [L, M], codes_r [L, N] -> idx, dist (float32, host_lenght), ncand."""
+    q, r = _f32(q), _f32(r)
+    cq = np.ascontiguousarray(codes_q, np.uint32)
+    cr = np.ascontiguousarray(codes_r, np.uint32)
+    idx = np.empty((q.shape[0], 2), np.uint32)
+    dist = np.empty((q.shape[0], 2), np.float32)
+    nc = np.empty((q.shape[0],), np.uint32)
+    _lib().oracle_lsh_query_f32(q.ctypes.data, r.ctypes.data, q.shape[0], r.shape[0], cq.shape[0], cq.ctypes.data,
+                                cr.ctypes.data, idx.ctypes.data, dist.ctypes.data, nc.ctypes.data, threads)
     return idx, dist, nc
 
 

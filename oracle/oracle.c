@@ -196,6 +196,26 @@ void oracle_lsh_codes(const float* x, int n, const float* H, int nbits, uint32_t
     }
 }
 
+/* Hash input of the float mode: clamp(rint(x * 2^k), -255, 255), exact in
+ * float (a power-of-two scale, then round-half-even as the GPU's rintf). */
+void oracle_lsh_quantize(const float* x, size_t count, int k, float* out) {
+    const float sc = ldexpf(1.0f, k);
+    for (size_t i = 0; i < count; i++) {
+        float v = nearbyintf(x[i] * sc);
+        out[i] = v < -255.0f ? -255.0f : (v > 255.0f ? 255.0f : v);
+    }
+}
+
+/* The largest k in [-100, 100] with max|x| * 2^k <= 255 (0 for all zeros). */
+int oracle_lsh_scale_exp(const float* x, size_t count) {
+    double mx = 0.0;
+    for (size_t i = 0; i < count; i++) mx = fabs((double)x[i]) > mx ? fabs((double)x[i]) : mx;
+    if (!(mx > 0.0)) return 0;
+    int k = -100;
+    while (k < 100 && ldexp(mx, k + 1) <= 255.0) k++;
+    return k;
+}
+
 static const uint32_t* g_sort_codes;
 static int cmp_code_idx(const void* a, const void* b) {
     uint32_t ia = *(const uint32_t*)a, ib = *(const uint32_t*)b;
@@ -237,6 +257,42 @@ void oracle_lsh_query(const float* q, const float* r, int q_n, int r_n, int L,
         idx2[2 * (size_t)i + 1] = i2;
         dist2[2 * (size_t)i] = d1 == UINT64_MAX ? UINT32_MAX : (uint32_t)d1;
         dist2[2 * (size_t)i + 1] = d2 == UINT64_MAX ? UINT32_MAX : (uint32_t)d2;
+        ncand[i] = nc;
+    }
+}
+
+/* The float mode's re-rank: the same candidate rule, distances by
+ * host_lenght (sequential FP32), top-2 by (distance, index). */
+void oracle_lsh_query_f32(const float* q, const float* r, int q_n, int r_n, int L, const uint32_t* codes_q,
+                          const uint32_t* codes_r, uint32_t* idx2, float* dist2, uint32_t* ncand, int threads) {
+#ifdef _OPENMP
+    if (threads < 1) threads = 1;
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4)
+#endif
+    for (int i = 0; i < q_n; i++) {
+        float d1 = FLT_MAX, d2 = FLT_MAX;
+        uint32_t i1 = UINT32_MAX, i2 = UINT32_MAX, nc = 0;
+        for (int j = 0; j < r_n; j++) {
+            int hit = 0;
+            for (int t = 0; t < L && !hit; t++)
+                hit = codes_q[(size_t)t * q_n + i] == codes_r[(size_t)t * r_n + j];
+            if (!hit) continue;
+            nc++;
+            const float d = host_lenght(q + (size_t)i * 128, r + (size_t)j * 128);
+            if (d < d1) {  /* rows come in increasing order: strict '<' keeps the lowest index */
+                d2 = d1;
+                i2 = i1;
+                d1 = d;
+                i1 = (uint32_t)j;
+            } else if (d < d2) {
+                d2 = d;
+                i2 = (uint32_t)j;
+            }
+        }
+        idx2[2 * (size_t)i] = i1;
+        idx2[2 * (size_t)i + 1] = i2;
+        dist2[2 * (size_t)i] = d1;
+        dist2[2 * (size_t)i + 1] = d2;
         ncand[i] = nc;
     }
 }

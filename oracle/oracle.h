@@ -7,6 +7,7 @@
  */
 #ifndef SIFT_ORACLE_H
 #define SIFT_ORACLE_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -45,6 +46,13 @@ void oracle_ratio(const float* sorted4, int q_n, float thr, uint32_t* matches);
 void oracle_lsh_codes(const float* x, int n, const float* H, int nbits, uint32_t* codes);
 /* Stable sort of (code, index) pairs; perm[k] = index of k-th smallest. */
 void oracle_lsh_sort(const uint32_t* codes, int n, uint32_t* perm);
+/* Float mode of the LSH index (data that is not all integers in [0,255]):
+ * hash input clamp(rint(x * 2^k), -255, 255); k = the largest in [-100, 100]
+ * with max|x| * 2^k <= 255; re-rank by host_lenght. */
+void oracle_lsh_quantize(const float* x, size_t count, int k, float* out);
+int oracle_lsh_scale_exp(const float* x, size_t count);
+void oracle_lsh_query_f32(const float* q, const float* r, int q_n, int r_n, int L, const uint32_t* codes_q,
+                          const uint32_t* codes_r, uint32_t* idx2, float* dist2, uint32_t* ncand, int threads);
 /* Exact 2-NN of each query restricted to the union over tables t of the base
  * rows whose code in table t equals the query's code.  codes_q[t*q_n+i],
  * codes_r[t*r_n+j].  ncand[i] = size of the candidate union. */

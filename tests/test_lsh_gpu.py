@@ -102,11 +102,109 @@ def test_lsh_gpu_entry_point(handle):
     ix.close()
 
 
-def test_lsh_rejects_non_integer(handle):
-    b = datagen.unit(100, seed=1)
+def check_float_index(ix, q, b, ratio=0.8, P=0):
+    """Float mode: the hashed rows are oracle.lsh_quantize(x, k) with k from
+    the base, codes and buckets as the integer path on those, and the re-rank
+    is the oracle's by host_lenght (sequential FP32) over the rows as given."""
+    L, nbits, n = ix.L, ix.nbits, b.shape[0]
+    fm, k = ix.hash_input()
+    assert fm and k == oracle.lsh_scale_exp(b)
+    H = ix.hyperplanes()
+    bq, qq = oracle.lsh_quantize(b, k), oracle.lsh_quantize(q, k)
+    cr, cq = oracle_tables(bq, H), oracle_tables(qq, H)
+    np.testing.assert_array_equal(ix.codes(q), cq)
+    np.testing.assert_array_equal(ix.codes(b), cr)
+    keys, rows = ix.buckets()
+    for t in range(L):
+        np.testing.assert_array_equal(rows[t * n:(t + 1) * n], oracle.lsh_sort(cr[t]))
+    if P:
+        ix.set_probes(P)
+        pc = ix.probe_codes(q)
+        for t in range(L):  # probes flip the bits of smallest |h . x| of the hashed integers
+            marg = np.abs(qq.astype(np.float64) @ H[t].astype(np.float64).T)
+            flips = pc[t, 1:] ^ cq[t][None]
+            np.testing.assert_array_equal(np.log2(flips).astype(np.int64).T, smallest_bits(marg, P))
+        cq = pc.reshape(L * (P + 1), -1)
+        cr = np.repeat(cr, P + 1, axis=0)
+    idx, dist, nc, mat = ix.query(q, ratio=ratio)
+    oi, od, _ = oracle.lsh_query_f32(q, b, cq, cr)
+    np.testing.assert_array_equal(idx, oi)
+    np.testing.assert_array_equal(dist, od)
+    want_nc = sum((cr[t][None, :] == cq[t][:, None]).sum(1) for t in range(cq.shape[0]))  # with repeats
+    np.testing.assert_array_equal(nc, want_nc.astype(np.uint32))
+    s4 = np.concatenate([od, np.where(oi == 0xFFFFFFFF, FMAX, oi.astype(np.float32))], 1)
+    want_m = oracle.ratio(s4, ratio)
+    want_m[oi[:, 0] == 0xFFFFFFFF] = 0xFFFFFFFF
+    np.testing.assert_array_equal(mat, want_m)
+    return idx
+
+
+@pytest.mark.parametrize("nbits,L,P", [(8, 2, 0), (12, 4, 0), (16, 8, 3), (24, 2, 0), (32, 1, 2)])
+def test_lsh_float_data(handle, nbits, L, P):
+    """The reference's own inputs (unit-norm floats, src/helper.cu:40-58, with
+    near-duplicate queries) build the float mode and equal the oracle's
+    float-mode LSH exactly: codes, buckets, probes, re-rank and matches."""
+    say("lsh float", nbits, L, P)
+    b = datagen.unit(8000, seed=71)
+    rng = np.random.default_rng(72)
+    q = (b[rng.integers(0, 8000, 200)] + rng.normal(0, 2e-3, (200, 128))).astype(np.float32)
+    q = np.concatenate([q, datagen.unit(100, seed=73)])
+    ix = LSHIndex(handle, b, nbits, L, seed=3 + nbits)
+    idx = check_float_index(ix, q, b, P=P)
+    if nbits <= 12:  # large buckets: the near duplicates find their source row
+        assert (idx[:200, 0] == oracle.host_brute(q[:200], b, threads=8)[:, 2]).mean() > 0.9
+    ix.close()
+
+
+def test_lsh_float_sift_div_512_and_half(handle):
+    """SIFT / 512 (src/test.cu:103) and fp16 inputs (the reference converts
+    float inputs to fp16, src/lsh_h.cu:748-780): float mode, exact against the
+    oracle on the values as given."""
+    b = datagen.base(6000, seed=74) / np.float32(512)
+    q = datagen.queries(b * 512, 250, seed=75) / np.float32(512)
+    ix = LSHIndex(handle, b, 12, 3, seed=5)
+    check_float_index(ix, q, b)
+    ix.close()
+    bh, qh = b.astype(np.float16), q.astype(np.float16)
+    ix = LSHIndex(handle, bh, 12, 3, seed=5)
+    check_float_index(ix, qh.astype(np.float32), bh.astype(np.float32))
+    ix.close()
+
+
+def test_lsh_integer_index_refuses_float_queries(handle):
+    b = datagen.base(3000, seed=76)
+    ix = LSHIndex(handle, b, 12, 2, seed=1)
+    assert ix.hash_input() == (False, 0)
     with pytest.raises(SiftError) as e:
-        LSHIndex(handle, b, 8, 1)
+        ix.query(datagen.queries(b, 20, seed=77) + np.float32(0.5))
     assert e.value.code == -5
+    ix.close()
+
+
+def test_lsh_gpu_float_data(handle):
+    """lsh_gpu with the reference caller's unit-norm data (type 1), and with an
+    integer base but non-integer queries (the float mode is then chosen for
+    the base too)."""
+    b = datagen.unit(6000, seed=78)
+    rng = np.random.default_rng(79)
+    q = (b[rng.integers(0, 6000, 150)] + rng.normal(0, 2e-3, (150, 128))).astype(np.float32)
+    for base, qq in ((b, q), (datagen.base(6000, seed=80), datagen.queries(datagen.base(6000, seed=80), 150,
+                                                                            seed=81) + np.float32(0.25))):
+        say("lsh_gpu float")
+        qd, bd = DeviceArray.from_numpy(qq), DeviceArray.from_numpy(base)
+        m = DeviceArray.empty((qq.shape[0],), np.uint32)
+        api.lsh_gpu(qd, bd, FLOAT_DEVICE, qq.shape[0], base.shape[0], m, 0.8, handle, 16, 4, LSH_HYPERPLANE, 12)
+        k = oracle.lsh_scale_exp(base)
+        ix = LSHIndex(handle, b[:64], 12, 4, seed=1234)  # the handle's seed: the same hyperplanes
+        H = ix.hyperplanes()
+        ix.close()
+        cr = oracle_tables(oracle.lsh_quantize(base, k), H)
+        cq = oracle_tables(oracle.lsh_quantize(qq, k), H)
+        oi, od, _ = oracle.lsh_query_f32(qq, base, cq, cr)
+        s4 = np.concatenate([od, np.where(oi == 0xFFFFFFFF, FMAX, oi.astype(np.float32))], 1)
+        want = oracle.ratio(s4, 0.8)
+        want[oi[:, 0] == 0xFFFFFFFF] = 0xFFFFFFFF
+        np.testing.assert_array_equal(m.numpy(), want)
 
 
 def test_lsh_rejects_bad_params(handle):
@@ -136,6 +234,18 @@ def test_lsh_save_load(handle, tmp_path):
         for x, y in zip(ref, jx.query(q)):
             np.testing.assert_array_equal(x, y)
         jx.close()
+    say("lsh save/load, float mode")
+    bf = datagen.unit(5000, seed=63)
+    qf = bf[:100] + np.float32(1e-3)
+    ix = LSHIndex(handle, bf, 12, 3, seed=6)
+    ref = ix.query(qf)
+    ix.save(tmp_path / "f.lsh")
+    ix.close()
+    jx = LSHIndex.load(handle, tmp_path / "f.lsh")
+    assert jx.hash_input() == (True, oracle.lsh_scale_exp(bf))
+    for x, y in zip(ref, jx.query(qf)):
+        np.testing.assert_array_equal(x, y)
+    jx.close()
     (tmp_path / "bad.lsh").write_bytes(b"not an index")
     with pytest.raises(SiftError):
         LSHIndex.load(handle, tmp_path / "bad.lsh")
@@ -259,3 +369,28 @@ def test_lsh_random_configs(handle, monkeypatch, seed):
             np.testing.assert_array_equal(idx, oi, err_msg="order %s" % order)
             np.testing.assert_array_equal(dist, np.where(oi == 0xFFFFFFFF, FMAX, od.astype(np.float32)))
         ix.close()
+
+
+def test_lsh_float_fallback(handle):
+    """A float-mode index with an F32 brute-force index as its exact fallback:
+    queries falling back get host_brute's answer (coarse keys, exact re-rank
+    from the F32 tiles); the rest keep the float LSH answer."""
+    from matching_sift_amd import COMPUTE_F32, BruteForceIndex
+    b = datagen.unit(6000, seed=91)
+    rng = np.random.default_rng(92)
+    q = (b[rng.integers(0, 6000, 120)] + rng.normal(0, 2e-3, (120, 128))).astype(np.float32)
+    q = np.concatenate([q, datagen.unit(80, seed=93)])
+    ix = LSHIndex(handle, b, 16, 2, seed=4)
+    bf = BruteForceIndex(handle, b, compute=COMPUTE_F32)
+    say("lsh float fallback")
+    lsh_idx, lsh_dist, nc, _ = ix.query(q)
+    idx, dist, nc2, mat, nfb = ix.query_fallback(q, bf, min_cand=20)
+    s = oracle.host_brute(q, b, threads=8)
+    low = nc < 20
+    assert nfb == int(low.sum()) and 0 < nfb < len(q)
+    np.testing.assert_array_equal(idx[low], s[low, 2:4].astype(np.uint32))
+    np.testing.assert_array_equal(dist[low], s[low, 0:2])
+    np.testing.assert_array_equal(idx[~low], lsh_idx[~low])
+    np.testing.assert_array_equal(dist[~low], lsh_dist[~low])
+    bf.close()
+    ix.close()

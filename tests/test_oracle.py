@@ -357,3 +357,51 @@ def test_host_brute_on_float_data_matches_numpy():
         assert (s[:30, 0] < 0.01).all()
         got = oracle.host_lenght(q, r, np.arange(q.shape[0]), s[:, 2].astype(np.int64))
         np.testing.assert_array_equal(got, s[:, 0])
+
+
+def test_lsh_scale_exp_and_quantize():
+    """Closed forms of the float mode's hash input: k is the largest integer
+    with max|x| 2^k <= 255 (checked on both sides of the boundary), and the
+    hashed values are numpy's round-half-even of x 2^k, clipped to [-255, 255]."""
+    for mx, k in ((255.0, 0), (256.0, -1), (127.5, 1), (127.6, 0), (1.0, 7), (0.5, 8), (1e-3, 17)):
+        x = np.zeros((1, 128), np.float32)
+        x[0, 5] = -mx
+        assert oracle.lsh_scale_exp(x) == k
+        assert np.float64(mx) * 2.0 ** k <= 255 < np.float64(mx) * 2.0 ** (k + 1)
+    assert oracle.lsh_scale_exp(np.zeros((3, 128), np.float32)) == 0
+    x = datagen.unit(50, seed=81) * np.float32(3.0)
+    k = oracle.lsh_scale_exp(x)
+    x[0, :4] = np.array([0.5, 1.5, 2.5, -0.5], np.float32) * np.float32(2.0 ** -k)  # exact halves once scaled
+    assert oracle.lsh_scale_exp(x) == k
+    want = np.clip(np.rint(x.astype(np.float64) * 2.0 ** k), -255, 255).astype(np.float32)
+    got = oracle.lsh_quantize(x, k)
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(got[0, :4], [0.0, 2.0, 2.0, -0.0])  # half to even
+    np.testing.assert_array_equal(oracle.lsh_quantize(np.full((1, 128), 1e6, np.float32), 0), 255.0)
+
+
+def test_lsh_query_f32_matches_numpy():
+    """The float mode's re-rank: over the same-bucket candidates of any table,
+    the two smallest host_lenght distances by (distance, index), equal to an
+    independent numpy restatement, including duplicated base rows."""
+    rng = np.random.default_rng(82)
+    r = datagen.unit(400, seed=83)
+    r[200:210] = r[0:10]  # exact duplicates: the lower index must win
+    q = (r[rng.integers(0, 400, 60)] + rng.normal(0, 1e-3, (60, 128))).astype(np.float32)
+    q[:5] = r[0:5]
+    L = 3
+    cr = rng.integers(0, 4, size=(L, 400)).astype(np.uint32)
+    cr[:, 200:210] = cr[:, 0:10]
+    cq = rng.integers(0, 4, size=(L, 60)).astype(np.uint32)
+    cq[:, :5] = cr[:, :5]
+    d = numpy_seq_dist(q, r)
+    idx, dist, nc = oracle.lsh_query_f32(q, r, cq, cr)
+    for i in range(60):
+        cand = np.nonzero((cr == cq[:, i:i + 1]).any(0))[0]
+        assert nc[i] == len(cand)
+        order = cand[np.lexsort((cand, d[i, cand]))][:2]
+        want = list(order) + [0xFFFFFFFF] * (2 - len(order))
+        np.testing.assert_array_equal(idx[i], want)
+        for k2, j in enumerate(order):
+            assert dist[i, k2] == d[i, j]
+    assert (idx[:5, 0] == np.arange(5)).all() and (dist[:5, 0] == 0).all()
