@@ -31,6 +31,7 @@ struct MergeArgs {
     int* bound_out;       // seeding mode: write the shared 2nd-best bound instead
     const int* coarse;    // float kinds: nonzero -> coarse keys (see Knn2Out::coarse)
     float slack;          // coarse keys: relative rounding bound of the kind
+    ExactSrc src;         // coarse keys: the rows of the exact re-rank
 };
 
 // Coarse keys (float kinds on data that is not all integers in [0,255]):
@@ -63,18 +64,14 @@ struct RerankArgs {
     float ratio;
 };
 
-// Exact re-rank of coarse keys, one wave per query (nothing to do when the
-// data was all integers in [0,255]): the EXACT_CAND smallest coarse (key,
-// row) over every slab partial (each lane a sorted list of its own entries,
-// then wave-wide minimum rounds), the sequential FP32 distance of each
-// candidate (one lane each), and the two smallest (distance, row) - the
-// result of host_brute restricted to the candidates.
-__global__ __launch_bounds__(256) void exact_rerank(RerankArgs a) {
-    if (*a.coarse == 0) return;
+// Exact re-rank of coarse keys for query q, one wave (the final merge runs it
+// when the data were not all integers in [0,255]): the EXACT_CAND smallest
+// coarse (key, row) over every slab partial (each lane a sorted list of its
+// own entries, then wave-wide minimum rounds), the sequential FP32 distance
+// of each candidate (one lane each), and the two smallest (distance, row) -
+// the result of host_brute restricted to the candidates.
+__device__ inline void exact_rerank_one(const RerankArgs& a, uint32_t q, int lane) {
     constexpr uint64_t NONE = ~0ull;
-    const int lane = threadIdx.x & 63;
-    const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= a.q_n) return;  // whole wave
     uint64_t l[EXACT_CAND];
 #pragma unroll
     for (int j = 0; j < EXACT_CAND; j++) l[j] = NONE;
@@ -120,7 +117,7 @@ __global__ __launch_bounds__(256) void exact_rerank(RerankArgs a) {
         m1 = min(m1, b1);
         m2 = n2;
     }
-    if (lane != 0) return;
+    if (lane != 0) return;  // lane 0 writes (the other lanes rejoin after it)
     const bool h1 = m1 != NONE, h2 = m2 != NONE;
     const float d1 = h1 ? __uint_as_float((uint32_t)(m1 >> 32)) : FLT_MAX;
     const float d2 = h2 ? __uint_as_float((uint32_t)(m2 >> 32)) : FLT_MAX;
@@ -202,8 +199,15 @@ __global__ __launch_bounds__(64 * MERGE_WAVES) void merge_top2(MergeArgs a) {
     using key_t = typename Key<KIND>::T;
     __shared__ uint4 red[MERGE_WAVES - 1][64];
     const bool coarse = a.coarse != nullptr && *a.coarse != 0;
-    if (coarse && !a.bound_out) return;  // the exact re-rank writes the outputs
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (coarse && !a.bound_out) {  // the exact re-rank of the block's 64 queries, one wave each
+        RerankArgs ra{a.coarse, a.part, a.q_n, a.q_stride, a.n_slabs, a.src, a.idx, a.dist, a.sorted, a.matches, a.ratio};
+        for (uint32_t i = 0; i < 64 / MERGE_WAVES; i++) {
+            const uint32_t q = blockIdx.x * 64u + (uint32_t)w * (64 / MERGE_WAVES) + i;
+            if (q < a.q_n) exact_rerank_one(ra, q, lane);
+        }
+        return;
+    }
     const uint32_t q = blockIdx.x * 64u + lane;
     key_t m1 = Key<KIND>::none(), m2 = Key<KIND>::none();
     uint32_t i1 = 0xFFFFFFFFu, i2 = 0xFFFFFFFFu;
@@ -388,6 +392,8 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
     }
     const bool float_kind = bs.kind == KIND_BF16 || bs.kind == KIND_F16 || bs.kind == KIND_F32;
     const int* coarse = float_kind ? out.coarse : nullptr;
+    if (coarse && (!out.src.q || (r_n && !out.src.r)))
+        return set_error(h, SIFT_ERR_INVALID, "coarse keys need the rows for the exact re-rank");
 
     auto mk_fused = [&](int l) {
         FusedArgs fa{};
@@ -427,6 +433,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         ma.bound_out = nullptr;
         ma.coarse = coarse;
         ma.slack = coarse_rel(bs.kind);
+        ma.src = out.src;
         return ma;
     };
     auto launch_merge = [&](const MergeArgs& ma) {
@@ -466,25 +473,6 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
     {
         ProfScope ps(h, PROF_MERGE);
         launch_merge(mk_merge(p0[n_launch]));
-        SIFT_HIP_TRY(h, hipGetLastError());
-    }
-    if (coarse) {  // does nothing on the device when the data was all integers in [0,255]
-        if (!out.src.q || (r_n && !out.src.r))
-            return set_error(h, SIFT_ERR_INVALID, "coarse keys need the rows for the exact re-rank");
-        ProfScope ps(h, PROF_MERGE);
-        RerankArgs ra;
-        ra.coarse = coarse;
-        ra.part = part;
-        ra.q_n = q_n;
-        ra.q_stride = q_stride;
-        ra.n_slabs = p0[n_launch];
-        ra.src = out.src;
-        ra.idx = out.idx;
-        ra.dist = out.dist;
-        ra.sorted = out.sorted;
-        ra.matches = out.matches;
-        ra.ratio = out.ratio;
-        exact_rerank<<<(q_n + 3) / 4, 256, 0, h->stream>>>(ra);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     return SIFT_OK;
