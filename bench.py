@@ -79,6 +79,10 @@ def lsh_sweep(args, ms, q, b, h, qd, bd):
     tables = [int(x) for x in args.lsh_tables.split(",")]
     outs = (ms.DeviceArray.empty((M, 2), np.uint32), ms.DeviceArray.empty((M, 2), np.float32),
             ms.DeviceArray.empty((M,), np.uint32), None)
+    # first launches of the build kernels (code loading) outside the timed builds
+    small = ms.DeviceArray.from_numpy(np.ascontiguousarray(b[:8192]))
+    ms.LSHIndex(h, small, 8, 2, seed=1, type_=ms.FLOAT_DEVICE).close()
+    h.sync()
     for nb in bits:
         for L in tables:
             h.sync()

@@ -34,8 +34,8 @@
 
 namespace sift {
 int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
-int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk,
-                     uint32_t** tv, uint32_t n, int bits, DevBuf& hist, DevBuf* levels);
+int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk, uint32_t** tv,
+                     uint32_t seg_n, uint32_t nseg, int bits, DevBuf& hist, DevBuf* levels, bool implicit_vals);
 }  // namespace sift
 
 using namespace sift;
@@ -47,6 +47,7 @@ struct sift_lsh_index {
     uint64_t seed = 0;
     std::vector<float> planes;  // [L][nbits][128]
     PackedSet hp;               // hyperplanes packed as B operands, 32 per table
+    DevBuf hp_i8;               // the same as int8 B operands ([L][4][64] x 16 B), then thr = -128 sum(h) [L][32]
     std::vector<sift_mlp*> mlps;  // learned hash: one network per table (owned by the caller,
                                   // or by the index when it was loaded from a file)
     bool owns_mlps = false;
@@ -84,6 +85,7 @@ LshScratch* scratch_of(sift_context* h) {
 }
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------ projection --
 // One wave per 32-point tile; every table's 32 hyperplanes in turn.
@@ -128,19 +130,166 @@ __global__ __launch_bounds__(256) void lsh_project(const uint4* __restrict__ x_t
     }
 }
 
-__global__ void iota_mod(uint32_t* __restrict__ v, uint32_t total, uint32_t n) {
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i < total) v[i] = (uint32_t)(i % n);
+// Build pass over integer data, reading each input row once.  A block of
+// PH_WAVES waves, one 32-row tile per wave.  Phase 1 reads the tile coalesced
+// (16 lanes per row, 8 dims per lane, four rows per step), checks that every
+// value is an integer in [0,255] (else *bad is set and the build redoes the
+// pass in float mode), writes the uint8 gather copy and the int32 squared
+// norms, and stages the rows as int8 x - 128 in LDS.  Phase 2 reads them back
+// in the int8 MFMA A layout (lane l: row l % 32, dims 32g + 16(l / 32) + 0..15
+// for g = 0..3).  Phase 3 reuses the LDS for the int8 hyperplanes of up to 8
+// tables at a time, shared by the block's waves: plane p of table t gives
+// bit p of the code where h.x < 0, i.e. h.(x - 128) < thr = -128 sum(h) (all
+// integers, exact).  It writes for every table t the key (t << b | code) at
+// t * n + row (the unsorted keys of the radix sort, whose values are the
+// positions within each table).
+// Code words of a 32-row x 32-plane accumulator tile (lane l: plane l % 32;
+// element j: rows 8(j/4) + (j%4) in lanes 0-31, that + 4 in lanes 32-63):
+// bit p of row r's word is acc(r, p) < th(p); lane r (< 32) receives row r's
+// word.  One ballot per element; the two halves of each go into their rows'
+// lanes by v_writelane.  The s_nop covers the wait states between the
+// compares that wrote the SGPRs and the v_writelane that reads them.
+__device__ inline uint32_t code_words(const i32x16& acc, int th) {
+    uint32_t w[32];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const uint64_t mb = __ballot(acc[j] < th);
+        w[2 * j] = (uint32_t)mb;
+        w[2 * j + 1] = (uint32_t)(mb >> 32);
+    }
+    uint32_t code = 0;
+    asm volatile(
+        "s_nop 4\n"
+        "v_writelane_b32 %0, %1, 0\n"
+        "v_writelane_b32 %0, %2, 4\n"
+        "v_writelane_b32 %0, %3, 1\n"
+        "v_writelane_b32 %0, %4, 5\n"
+        "v_writelane_b32 %0, %5, 2\n"
+        "v_writelane_b32 %0, %6, 6\n"
+        "v_writelane_b32 %0, %7, 3\n"
+        "v_writelane_b32 %0, %8, 7\n"
+        "v_writelane_b32 %0, %9, 8\n"
+        "v_writelane_b32 %0, %10, 12\n"
+        "v_writelane_b32 %0, %11, 9\n"
+        "v_writelane_b32 %0, %12, 13\n"
+        "v_writelane_b32 %0, %13, 10\n"
+        "v_writelane_b32 %0, %14, 14\n"
+        "v_writelane_b32 %0, %15, 11\n"
+        "v_writelane_b32 %0, %16, 15\n"
+        "v_writelane_b32 %0, %17, 16\n"
+        "v_writelane_b32 %0, %18, 20\n"
+        "v_writelane_b32 %0, %19, 17\n"
+        "v_writelane_b32 %0, %20, 21\n"
+        "v_writelane_b32 %0, %21, 18\n"
+        "v_writelane_b32 %0, %22, 22\n"
+        "v_writelane_b32 %0, %23, 19\n"
+        "v_writelane_b32 %0, %24, 23\n"
+        "v_writelane_b32 %0, %25, 24\n"
+        "v_writelane_b32 %0, %26, 28\n"
+        "v_writelane_b32 %0, %27, 25\n"
+        "v_writelane_b32 %0, %28, 29\n"
+        "v_writelane_b32 %0, %29, 26\n"
+        "v_writelane_b32 %0, %30, 30\n"
+        "v_writelane_b32 %0, %31, 27\n"
+        "v_writelane_b32 %0, %32, 31\n"
+        : "+v"(code)
+        : "s"(w[0]), "s"(w[1]), "s"(w[2]), "s"(w[3]), "s"(w[4]), "s"(w[5]), "s"(w[6]), "s"(w[7]), "s"(w[8]), "s"(w[9]), "s"(w[10]), "s"(w[11]), "s"(w[12]), "s"(w[13]), "s"(w[14]), "s"(w[15]), "s"(w[16]), "s"(w[17]), "s"(w[18]), "s"(w[19]), "s"(w[20]), "s"(w[21]), "s"(w[22]), "s"(w[23]), "s"(w[24]), "s"(w[25]), "s"(w[26]), "s"(w[27]), "s"(w[28]), "s"(w[29]), "s"(w[30]), "s"(w[31]));
+    return code;
 }
 
-// Queries whose candidate count is below min_cand (order of the list is
-// arbitrary; the host sorts it).
+constexpr int PH_WAVES = 8;
+constexpr int PH_ROW = 9;         // uint4 per staged row: 8 of data + 1 of padding (conflict-free reads)
+constexpr int PH_TABLE = 4 * 64;  // uint4 of one table's int8 plane groups
+
+template <int TYPE>
+__global__ __launch_bounds__(64 * PH_WAVES) __attribute__((amdgpu_waves_per_eu(6))) void lsh_pack_hash(const void* __restrict__ in, uint32_t n,
+                                                               uint32_t n_tiles, const uint4* __restrict__ planes,
+                                                               const int* __restrict__ thr, int L, int b,
+                                                               uint8_t* __restrict__ u8_out,
+                                                               int* __restrict__ nrm_out, int* __restrict__ bad,
+                                                               uint32_t* __restrict__ keys) {
+    static_assert(PH_WAVES * 32 * PH_ROW >= 8 * PH_TABLE, "the planes of 8 tables fit the staging area");
+    __shared__ uint4 lds[PH_WAVES * 32 * PH_ROW];
+    __shared__ int lthr[8 * 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t tile = blockIdx.x * PH_WAVES + (uint32_t)wave;
+    const bool active = tile < n_tiles;  // idle waves still take part in the block's barriers
+    const uint32_t base = tile * 32;
+    uint4 a[4];
+    if (active) {
+        uint4* st = lds + wave * 32 * PH_ROW;
+        // phase 1: step k covers rows base + 4k .. base + 4k + 3; lane = 16 (row % 4) + chunk
+        const int c = lane & 15, rq = lane >> 4;
+        bool ok = true;
+#pragma unroll
+        for (int k0 = 0; k0 < 8; k0 += 4) {  // two batches of four steps, each batch's loads issued together
+            float v[4][8];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {  // padding rows read row n - 1
+                const uint32_t row = base + 4 * (k0 + k) + rq;
+                load8_src(in, TYPE, row < n ? row : n - 1, c, v[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                // rows past n repeat row n - 1 and write its (identical) bytes and norm again
+                const int r = 4 * (k0 + k) + rq;
+                const uint32_t grow = base + r < n ? base + r : n - 1;
+                uint32_t q[2] = {0u, 0u};
+#pragma unroll
+                for (int i = 0; i < 8; i++) {  // round and clamp to a byte, then check the byte is the value
+                    const float x = v[k][i];
+                    q[i >> 2] = __builtin_amdgcn_cvt_pk_u8_f32(x, i & 3, q[i >> 2]);
+                    ok &= x == (float)((q[i >> 2] >> (8 * (i & 3))) & 255u);
+                }
+                int sq = (int)__builtin_amdgcn_udot4(q[0], q[0], __builtin_amdgcn_udot4(q[1], q[1], 0u, false), false);
+                reinterpret_cast<uint2*>(st)[r * 2 * PH_ROW + c] = make_uint2(q[0] ^ 0x80808080u, q[1] ^ 0x80808080u);
+                // sum over the row's 16 lanes (one DPP row): rotations by 8, 4, 2, 1
+                sq += __builtin_amdgcn_update_dpp(0, sq, 0x128, 0xF, 0xF, false);
+                sq += __builtin_amdgcn_update_dpp(0, sq, 0x124, 0xF, 0xF, false);
+                sq += __builtin_amdgcn_update_dpp(0, sq, 0x122, 0xF, 0xF, false);
+                sq += __builtin_amdgcn_update_dpp(0, sq, 0x121, 0xF, 0xF, false);
+                reinterpret_cast<uint2*>(u8_out)[(size_t)grow * 16 + c] = make_uint2(q[0], q[1]);
+                if (c == 0) nrm_out[grow] = sq;
+            }
+            // the batch's checks done here (its values dead), its loads before the next batch's
+            int okv = ok;
+            asm volatile("" : "+v"(okv)::"memory");
+            ok = okv != 0;
+        }
+        const unsigned long long m = __ballot(!ok);
+        if (m && lane == __ffsll((long long)m) - 1) atomicOr(bad, 1);
+        // phase 2: the wave reads back what all its lanes wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int r = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int g = 0; g < 4; g++) a[g] = st[r * PH_ROW + 2 * g + h];
+    }
+    const uint32_t row = base + (uint32_t)(lane & 31);
+    for (int t0 = 0; t0 < L; t0 += 8) {
+        const int nt = min(8, L - t0);
+        __syncthreads();  // the staging area (or the previous planes) is no longer read
+        for (int e = threadIdx.x; e < nt * PH_TABLE; e += 64 * PH_WAVES) lds[e] = planes[(size_t)t0 * PH_TABLE + e];
+        if (threadIdx.x < nt * 32) lthr[threadIdx.x] = thr[t0 * 32 + threadIdx.x];
+        __syncthreads();
+        if (!active) continue;
+        for (int k = 0; k < nt; k++) {
+            const int t = t0 + k;
+            i32x16 acc = i32x16{};
+#pragma unroll
+            for (int g = 0; g < 4; g++) acc = KT<KIND_I8>::mma(a[g], lds[k * PH_TABLE + g * 64 + lane], acc);
+            const int th = lthr[k * 32 + (lane & 31)];  // accumulator column = plane lane % 32
+            const uint32_t code = code_words(acc, th);
+            if (lane < 32 && row < n) keys[(size_t)t * n + row] = b < 32 ? (((uint32_t)t << b) | code) : code;
+        }
+    }
+}
+
 // Queries the LSH answer does not settle: fewer than min_cand candidates, or
 // (accept_ratio > 0) a best candidate that does not pass the ratio test
 // sqrt(d1)/sqrt(d2) < accept_ratio against the next one (IEEE sqrt and
 // division, as the match output).
-// Queries the LSH answer does not settle: fewer than min_cand candidates, or
-// (accept_ratio > 0) a best candidate failing the ratio test against the next.
 __device__ inline bool needs_exact(const uint32_t* nc, const float* d, uint32_t q, uint32_t min_cand,
                                    float accept_ratio) {
     bool low = nc[q] < min_cand;
@@ -629,7 +778,7 @@ int rerank_units(sift_context* h, const sift_lsh_index* ix, LshScratch* sc, cons
     int tbits = 0;
     while ((1 << tbits) < ix->L) tbits++;
     const int bits = std::min(32, ix->nbits + tbits);
-    SIFT_TRY(radix_sort_pairs(h, &k, &v, &tk, &tv, nu, bits, sc->uhist, sc->ulevels));
+    SIFT_TRY(radix_sort_pairs(h, &k, &v, &tk, &tv, nu, 1, bits, sc->uhist, sc->ulevels, false));
     auto* part = static_cast<uint4*>(sc->upart.p);
     auto* cnt = static_cast<uint32_t*>(sc->ucnt.p);
     lsh_rerank_units<<<(nu + 3) / 4, 256, 0, h->stream>>>(a, v, nu, part, cnt);
@@ -643,7 +792,7 @@ void free_index(sift_lsh_index* ix) {
     if (ix->owns_mlps)
         for (sift_mlp* m : ix->mlps) sift_mlp_destroy(m);
     for (DevBuf* b : {&ix->keys, &ix->vals, &ix->offs, &ix->base_u8, &ix->base_nrm, &ix->base_f32, &ix->scratch_k,
-                      &ix->scratch_v, &ix->hist, &ix->hp.tiles, &ix->hp.norm, &ix->hp.norm_lane})
+                      &ix->scratch_v, &ix->hist, &ix->hp.tiles, &ix->hp.norm, &ix->hp.norm_lane, &ix->hp_i8})
         release(*b);
     for (auto& l : ix->levels) release(l);
     delete ix;
@@ -684,40 +833,54 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
 }
 
 // Sum over buckets of size^2 (= sum over sorted positions of the size of the
-// bucket holding them): the thread at each bucket's first position finds
-// the bucket's end by a galloping search (a few loads for small buckets),
-// the rest only compare with their left neighbour.  Grid-stride, one 64-bit
-// atomic per block.
-__global__ __launch_bounds__(256) void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n, uint32_t L,
-                                                 unsigned long long* __restrict__ acc) {
+// bucket holding them), one table per grid row: the thread at each bucket's
+// first position finds the bucket's end by a galloping search (a few loads
+// for small buckets), the rest only compare with their left neighbour.
+// Grid-stride; each block writes its partial sum to part[block] (the host
+// adds them: no atomics, the same total on every run).
+__device__ inline void block_sum_u64(unsigned long long s, unsigned long long* part) {
     __shared__ unsigned long long wsum[4];
-    const size_t total = (size_t)L * n;
-    unsigned long long s = 0;
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < total; p += (size_t)gridDim.x * blockDim.x) {
-        const uint32_t t0 = (uint32_t)(p / n) * n, t1 = t0 + n, pp = (uint32_t)p;
-        const uint32_t k = keys[pp];
-        if (pp == t0 || keys[pp - 1] != k) {  // first position of its bucket
-            uint32_t lo = pp, hi = t1;  // keys[lo] == k
-            for (uint64_t d = 1;; d <<= 1) {
-                const uint64_t nx = (uint64_t)pp + d;
-                if (nx >= t1 || keys[nx] != k) {
-                    hi = (uint32_t)min<uint64_t>(nx, t1);
-                    break;
-                }
-                lo = (uint32_t)nx;
-            }
-            const unsigned long long len = upper_bound_u32(keys, lo + 1, hi, k) - pp;
-            s += len * len;
-        }
-    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long b = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (b) atomicAdd(acc, b);
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(256) void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n,
+                                                 unsigned long long* __restrict__ part) {
+    const uint32_t* tk = keys + (size_t)blockIdx.y * n;  // this table's sorted keys
+    unsigned long long s = 0;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t k = tk[p];
+        if (p == 0 || tk[p - 1] != k) {  // first position of its bucket
+            uint32_t lo = p, hi = n;  // tk[lo] == k
+            for (uint64_t d = 1;; d <<= 1) {
+                const uint64_t nx = (uint64_t)p + d;
+                if (nx >= n || tk[nx] != k) {
+                    hi = (uint32_t)min<uint64_t>(nx, n);
+                    break;
+                }
+                lo = (uint32_t)nx;
+            }
+            const unsigned long long len = upper_bound_u32(tk, lo + 1, hi, k) - p;
+            s += len * len;
+        }
     }
+    block_sum_u64(s, part);
+}
+
+// The same sum from the bucket starts ([L][2^b + 1] positions).
+__global__ __launch_bounds__(256) void bucket_sq_offs(const uint32_t* __restrict__ offs, int b, uint32_t L,
+                                                      unsigned long long* __restrict__ part) {
+    const uint64_t nb = (uint64_t)L << b;
+    unsigned long long s = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t o = (i >> b) * ((1ull << b) + 1) + (i & ((1ull << b) - 1));
+        const unsigned long long len = offs[o + 1] - offs[o];
+        s += len * len;
+    }
+    block_sum_u64(s, part);
 }
 
 // The hyperplanes of every table as MFMA B operands: 32 rows per table,
@@ -736,6 +899,26 @@ int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
     if (rc == SIFT_OK)
         rc = pack_rows(h, hp_dev.p, SIFT_FLOAT_DEVICE, (uint32_t)L * 32, (uint32_t)L * 32, KIND_BF16, PACK_PLAIN,
                        ix->hp, flag);
+    // int8 B operands for the build pass (the planes are integers in [-127,127]):
+    // lane l holds plane l % 32, dims 32g + 16(l / 32) + 0..15 of group g
+    const size_t plane_bytes = (size_t)L * PH_TABLE * 16;
+    std::vector<int8_t> p8(plane_bytes + (size_t)L * 32 * 4, 0);
+    std::vector<int> thr((size_t)L * 32);
+    for (int t = 0; t < L; t++)
+        for (int p = 0; p < 32; p++) {
+            int sum = 0;
+            for (int d = 0; d < 128; d++) {
+                const int v = (int)hp[((size_t)t * 32 + p) * 128 + d];
+                sum += v;
+                const int g = d / 32, half = (d / 16) & 1, lane = p + 32 * half;
+                p8[(((size_t)t * 4 + g) * 64 + lane) * 16 + (d & 15)] = (int8_t)v;
+            }
+            thr[t * 32 + p] = -128 * sum;
+        }
+    std::memcpy(p8.data() + plane_bytes, thr.data(), thr.size() * 4);
+    if (rc == SIFT_OK) rc = reserve(h, ix->hp_i8, p8.size());
+    if (rc == SIFT_OK && hipMemcpyAsync(ix->hp_i8.p, p8.data(), p8.size(), hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        rc = set_error(h, SIFT_ERR_HIP, "copy hyperplanes");
     if (rc == SIFT_OK && hipStreamSynchronize(h->stream) != hipSuccess) rc = set_error(h, SIFT_ERR_HIP, "sync");
     release(hp_dev);
     return rc;
@@ -747,10 +930,15 @@ int pack_planes(sift_context* h, sift_lsh_index* ix, int* flag) {
 int build_offsets(sift_context* h, sift_lsh_index* ix) {
     ix->cand_per_lookup = 0.0;
     if (ix->n == 0) return SIFT_OK;
-    SIFT_TRY(reserve(h, ix->hist, 8));
-    auto* acc = static_cast<unsigned long long*>(ix->hist.p);
-    SIFT_HIP_TRY(h, hipMemsetAsync(acc, 0, 8, h->stream));
     const size_t total = (size_t)ix->L * ix->n;
+    dim3 grid;
+    if (ix->nbits <= OFFS_MAX_BITS)
+        grid = dim3((unsigned)std::min<uint64_t>((((uint64_t)ix->L << ix->nbits) + 255) / 256, 256));
+    else
+        grid = dim3((unsigned)std::min<size_t>((ix->n + 255) / 256, std::max(1, 1024 / ix->L)), (unsigned)ix->L);
+    const size_t nparts = (size_t)grid.x * grid.y;
+    SIFT_TRY(reserve(h, ix->hist, nparts * 8));
+    auto* part = static_cast<unsigned long long*>(ix->hist.p);
     {
         ProfScope ps(h, PROF_LSH_SORT);  // bucket structures count with the sort
         if (ix->nbits <= OFFS_MAX_BITS) {
@@ -760,14 +948,18 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
                 static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L,
                 static_cast<uint32_t*>(ix->offs.p));
             SIFT_HIP_TRY(h, hipGetLastError());
+            bucket_sq_offs<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->offs.p), ix->nbits,
+                                                        (uint32_t)ix->L, part);
+        } else {
+            bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, part);
         }
-        const unsigned grid = (unsigned)std::min<size_t>((total + 255) / 256, 2048);
-        bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, (uint32_t)ix->L, acc);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
-    unsigned long long sum = 0;
-    SIFT_HIP_TRY(h, hipMemcpyAsync(&sum, acc, 8, hipMemcpyDeviceToHost, h->stream));
+    std::vector<unsigned long long> sums(nparts);
+    SIFT_HIP_TRY(h, hipMemcpyAsync(sums.data(), part, nparts * 8, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
+    unsigned long long sum = 0;
+    for (unsigned long long v : sums) sum += v;
     ix->cand_per_lookup = (double)sum / (double)total;
     return SIFT_OK;
 }
@@ -786,17 +978,75 @@ int hash_scale_exp(float mx) {
     return std::max(-100, std::min(100, k));
 }
 
+// The fused integer build pass (lsh_pack_hash) over the index's base rows.
+int pack_hash_launch(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag, uint32_t* keys) {
+    const uint32_t n = ix->n, n_tiles = (n + 31) / 32;
+    const unsigned grid = (n_tiles + PH_WAVES - 1) / PH_WAVES;
+    const auto* pl = static_cast<const uint4*>(ix->hp_i8.p);
+    const auto* th = reinterpret_cast<const int*>(pl + (size_t)ix->L * PH_TABLE);
+    auto* u8 = static_cast<uint8_t*>(ix->base_u8.p);
+    auto* nrm = static_cast<int*>(ix->base_nrm.p);
+    const dim3 blk(64 * PH_WAVES);
+    switch (type) {
+        case SIFT_FLOAT_HOST:
+        case SIFT_FLOAT_DEVICE:
+            lsh_pack_hash<SIFT_FLOAT_DEVICE><<<grid, blk, 0, h->stream>>>(r, n, n_tiles, pl, th, ix->L, ix->nbits,
+                                                                         u8, nrm, flag, keys);
+            break;
+        case SIFT_HALF_DEVICE:
+            lsh_pack_hash<SIFT_HALF_DEVICE><<<grid, blk, 0, h->stream>>>(r, n, n_tiles, pl, th, ix->L, ix->nbits,
+                                                                        u8, nrm, flag, keys);
+            break;
+        case SIFT_BF16_DEVICE:
+            lsh_pack_hash<SIFT_BF16_DEVICE><<<grid, blk, 0, h->stream>>>(r, n, n_tiles, pl, th, ix->L, ix->nbits,
+                                                                        u8, nrm, flag, keys);
+            break;
+        case SIFT_U8_DEVICE:
+            lsh_pack_hash<SIFT_U8_DEVICE><<<grid, blk, 0, h->stream>>>(r, n, n_tiles, pl, th, ix->L, ix->nbits,
+                                                                      u8, nrm, flag, keys);
+            break;
+        default: return set_error(h, SIFT_ERR_INVALID, "unknown input type");
+    }
+    SIFT_HIP_TRY(h, hipGetLastError());
+    return SIFT_OK;
+}
+
 // flag: the handle's LSH flag words ([0] exactness, [1] scratch, [3] max |x|).
 int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag, bool force_float) {
     const uint32_t r_n = ix->n;
     const int L = ix->L, nbits = ix->nbits;
-    int tbits = 0;
-    while ((1 << tbits) < L) tbits++;
     int rc = SIFT_OK;
     do {
         const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
+        const size_t total = (size_t)r_n * L;
+        for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
+            if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
+        if (rc != SIFT_OK) break;
+        auto* keys = static_cast<uint32_t*>(ix->keys.p);
+        auto* vals = static_cast<uint32_t*>(ix->vals.p);
+        auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
+        auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
+        // hyperplanes over integer data: one pass writes the gather copy and the
+        // unsorted keys of every table
+        const bool fused = ix->hash_kind != SIFT_LSH_LEARNED;
+        bool hashed = false;
         bool float_mode = force_float;
-        if (!float_mode) {  // integer data: uint8 gather copy, exactness check, plain tiles
+        if (!float_mode && fused) {
+            if ((rc = reserve(h, ix->base_u8, std::max<size_t>((size_t)r_n * 128, 1))) != SIFT_OK) break;
+            if ((rc = reserve(h, ix->base_nrm, std::max<size_t>((size_t)r_n * 4, 1))) != SIFT_OK) break;
+            if (r_n) {
+                ProfScope ps(h, PROF_LSH_PROJ);
+                if ((rc = pack_hash_launch(h, ix, r, type, flag, keys)) != SIFT_OK) break;
+            }
+            int bad = 0;
+            if (hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+                hipStreamSynchronize(h->stream) != hipSuccess) {
+                rc = set_error(h, SIFT_ERR_HIP, "build flag");
+                break;
+            }
+            float_mode = bad != 0;
+            hashed = !float_mode;
+        } else if (!float_mode) {  // learned hash: uint8 gather copy, exactness check, plain tiles
             {
                 ProfScope ps(h, PROF_PACK);
                 if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
@@ -835,26 +1085,16 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
             if ((rc = to_f32(h, r, type, r_n, ix->base_f32)) != SIFT_OK) break;
             if ((rc = pack_quant(h, r, type, r_n, rows_pad, ix->qexp, h->b_set, flag + 1)) != SIFT_OK) break;
         }
-        const size_t total = (size_t)r_n * L;
-        for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
-            if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
-        if (rc != SIFT_OK) break;
-        auto* keys = static_cast<uint32_t*>(ix->keys.p);
-        auto* vals = static_cast<uint32_t*>(ix->vals.p);
-        auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
-        auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
-        {
-            ProfScope ps(h, PROF_LSH_PROJ);
-            if ((rc = hash_codes(h, ix, h->b_set, r_n, keys, r_n, true)) != SIFT_OK) break;
+        if (!hashed) {
+            {
+                ProfScope ps(h, PROF_LSH_PROJ);
+                if ((rc = hash_codes(h, ix, h->b_set, r_n, keys, r_n, true)) != SIFT_OK) break;
+            }
         }
-        if (total) {
-            iota_mod<<<(unsigned)((total + 255) / 256), 256, 0, h->stream>>>(vals, (uint32_t)total, r_n);
-            if (hipGetLastError() != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "iota"); break; }
-        }
-        {
+        {  // each table sorted in its own segment; the row ids come from positions
             ProfScope ps(h, PROF_LSH_SORT);
-            if ((rc = radix_sort_pairs(h, &keys, &vals, &tk, &tv, (uint32_t)total, nbits + tbits, ix->hist,
-                                       ix->levels)) != SIFT_OK)
+            if ((rc = radix_sort_pairs(h, &keys, &vals, &tk, &tv, r_n, (uint32_t)L, nbits, ix->hist, ix->levels,
+                                       true)) != SIFT_OK)
                 break;
         }
         // keep the sorted arrays in ix->keys / ix->vals

@@ -3,7 +3,8 @@
 //
 // Replaces the reference's thrust::sort with an indirect comparator
 // (src/lsh_h.cu:587-591) followed by a host run-length pass (:598-620).  Here
-// the whole table set is sorted on the device in ceil(bits/8) passes of:
+// every table is sorted on the device, each in its own segment of the pair
+// arrays, in ceil(bits/8) passes of:
 //   hist    : per-block 256-bin digit histogram (LDS atomics), bin-major
 //   scan    : exclusive scan of the histogram -> global digit offsets
 //   scatter : stable scatter; ranks among equal digits come from wave-wide
@@ -19,33 +20,54 @@ constexpr int RS_I = 16;             // keys per thread
 constexpr int RS_CHUNK = RS_T * RS_I;  // keys per block
 constexpr int RS_W = RS_T / 64;      // waves per block
 constexpr int RS_SUB = RS_CHUNK / RS_W;  // keys per wave, walked in order
+constexpr int RS_STEPS = RS_SUB / 64;  // keys per lane
 
-__global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t n,
-                                                int shift, uint32_t* __restrict__ hist,
-                                                uint32_t nblk) {
+// The pairs form nseg segments of seg_n consecutive elements, each sorted on
+// its own (segment s keeps positions [s seg_n, (s + 1) seg_n)); blocks never
+// straddle segments: block blk covers chunk blk % nbt of segment blk / nbt,
+// nbt = ceil(seg_n / RS_CHUNK).  Digit counts are laid out [s][digit][chunk],
+// so one exclusive scan over them yields each (segment, digit, chunk)'s
+// output start, inside its segment.
+struct SegChunk {
+    size_t base;   // first element
+    uint32_t m;    // elements in the chunk
+    uint32_t hrow; // s * 256 * nbt + chunk: add digit * nbt for the count slot
+};
+
+__device__ inline SegChunk seg_chunk(uint32_t blk, uint32_t seg_n, uint32_t nbt) {
+    const uint32_t s = blk / nbt, j = blk % nbt;
+    SegChunk c;
+    c.base = (size_t)s * seg_n + (size_t)j * RS_CHUNK;
+    c.m = min((uint32_t)RS_CHUNK, seg_n - j * (uint32_t)RS_CHUNK);
+    c.hrow = s * 256u * nbt + j;
+    return c;
+}
+
+__global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t seg_n, uint32_t nbt,
+                                                int shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * RS_CHUNK;
+    const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
 #pragma unroll
     for (int i = 0; i < RS_I; i++) {
-        const size_t idx = base + (size_t)i * RS_T + threadIdx.x;
-        if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & 255u], 1u);
+        const uint32_t idx = (uint32_t)i * RS_T + threadIdx.x;
+        if (idx < c.m) atomicAdd(&cnt[(keys[c.base + idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * nblk + blockIdx.x] = cnt[threadIdx.x];
+    hist[c.hrow + (size_t)threadIdx.x * nbt] = cnt[threadIdx.x];
 }
 
-// Stable scatter of one 4096-element chunk: ranks come from wave-wide
-// ballots, the chunk is first ordered by digit in LDS, then written out with
-// consecutive threads on consecutive positions of each digit's run (whole
-// runs instead of scattered 4-byte writes).
+// Stable scatter of one chunk: ranks come from wave-wide ballots, the chunk
+// is first ordered by digit in LDS, then written out with consecutive
+// threads on consecutive positions of each digit's run (whole runs instead
+// of scattered 4-byte writes).  vin == nullptr: the value of each element is
+// its position in its segment.
 __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ kin,
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
-                                                   uint32_t* __restrict__ vout, uint32_t n,
-                                                   int shift, const uint32_t* __restrict__ offs,
-                                                   uint32_t nblk) {
+                                                   uint32_t* __restrict__ vout, uint32_t seg_n, uint32_t nbt,
+                                                   int shift, const uint32_t* __restrict__ offs) {
     __shared__ uint32_t sk[RS_CHUNK], sv[RS_CHUNK];
     __shared__ uint32_t wh[RS_W][256];  // per-wave digit counts, then local cursors
     __shared__ uint32_t lstart[256], gstart[256], wsum[RS_W];
@@ -53,12 +75,17 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k < RS_W; k++) wh[k][threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * RS_CHUNK;
-    const size_t sub = base + (size_t)w * RS_SUB;
-    for (int step = 0; step < RS_SUB / 64; step++) {
-        const size_t idx = sub + (size_t)step * 64 + lane;
-        if (idx < n) atomicAdd(&wh[w][(kin[idx] >> shift) & 255u], 1u);
+    const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
+    const uint32_t sub = (uint32_t)w * RS_SUB;  // the wave's part of the chunk
+    uint32_t key[RS_STEPS];  // read once, kept for the ranking pass
+#pragma unroll
+    for (int step = 0; step < RS_STEPS; step++) {
+        const uint32_t idx = sub + (uint32_t)step * 64 + lane;
+        key[step] = idx < c.m ? kin[c.base + idx] : 0u;
     }
+#pragma unroll
+    for (int step = 0; step < RS_STEPS; step++)
+        if (sub + (uint32_t)step * 64 + lane < c.m) atomicAdd(&wh[w][(key[step] >> shift) & 255u], 1u);
     __syncthreads();
     {  // per digit: start of its run in the chunk (scan over digits), this
        // block's global offset, and each wave's local cursor
@@ -78,23 +105,24 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
         for (int k = 0; k < w; k++) pre += wsum[k];
         const uint32_t ls = pre + x - tot;
         lstart[bin] = ls;
-        gstart[bin] = offs[(size_t)bin * nblk + blockIdx.x];
+        gstart[bin] = offs[c.hrow + (size_t)bin * nbt];
         uint32_t run = ls;
 #pragma unroll
         for (int k = 0; k < RS_W; k++) {
-            const uint32_t c = wh[k][bin];
+            const uint32_t cw = wh[k][bin];
             wh[k][bin] = run;
-            run += c;
+            run += cw;
         }
     }
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int step = 0; step < RS_SUB / 64; step++) {
-        const size_t idx = sub + (size_t)step * 64 + lane;
-        const bool valid = idx < n;
-        const uint32_t key = valid ? kin[idx] : 0u;
-        const uint32_t val = valid ? vin[idx] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
+    const uint32_t vbase = (uint32_t)(c.base % seg_n);
+#pragma unroll
+    for (int step = 0; step < RS_STEPS; step++) {
+        const uint32_t idx = sub + (uint32_t)step * 64 + lane;
+        const bool valid = idx < c.m;
+        const uint32_t val = !valid ? 0u : vin ? vin[c.base + idx] : vbase + idx;
+        const uint32_t d = (key[step] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
@@ -105,19 +133,18 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
         const uint32_t pos = wh[w][d];
         if (valid) {
-            sk[pos + rank] = key;
+            sk[pos + rank] = key[step];
             sv[pos + rank] = val;
         }
         // lowest lane of each digit group advances that digit's cursor
         if (valid && (peers & lt) == 0) wh[w][d] = pos + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    const uint32_t m = (uint32_t)min<size_t>((size_t)RS_CHUNK, n - base);
-    for (uint32_t i = threadIdx.x; i < m; i += RS_T) {
-        const uint32_t key = sk[i];
-        const uint32_t d = (key >> shift) & 255u;
+    for (uint32_t i = threadIdx.x; i < c.m; i += RS_T) {
+        const uint32_t k = sk[i];
+        const uint32_t d = (k >> shift) & 255u;
         const uint32_t pos = gstart[d] + (i - lstart[d]);
-        kout[pos] = key;
+        kout[pos] = k;
         vout[pos] = sv[i];
     }
 }
@@ -182,19 +209,26 @@ int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, 
     return SIFT_OK;
 }
 
-// Stable sort of n (key, value) pairs by the low `bits` of key.  Sorted pairs
-// end in (*keys, *vals); the other buffers are scratch of the same size.
-int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk,
-                     uint32_t** tv, uint32_t n, int bits, DevBuf& hist, DevBuf* levels) {
-    if (n == 0) return SIFT_OK;
-    const uint32_t nblk = (n + RS_CHUNK - 1) / RS_CHUNK;
+// Stable sort of the (key, value) pairs of each of nseg segments of seg_n
+// elements by the low `bits` of key (a segment's pairs stay in its
+// positions).  implicit_vals: the values are not read, each is its
+// element's position in its segment (*vals is then only scratch).  Sorted
+// pairs end in (*keys, *vals); the other buffers are scratch of the same size.
+int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk, uint32_t** tv,
+                     uint32_t seg_n, uint32_t nseg, int bits, DevBuf& hist, DevBuf* levels, bool implicit_vals) {
+    if (seg_n == 0 || nseg == 0) return SIFT_OK;
+    if ((uint64_t)seg_n * nseg >= (1ull << 32)) return set_error(h, SIFT_ERR_UNSUPPORTED, "sort: too many pairs");
+    const uint32_t nbt = (seg_n + RS_CHUNK - 1) / RS_CHUNK;
+    const uint64_t nblk = (uint64_t)nbt * nseg;
+    if (nblk * 256 >= (1ull << 32)) return set_error(h, SIFT_ERR_UNSUPPORTED, "sort: too many chunks");
     SIFT_TRY(reserve(h, hist, (size_t)nblk * 256 * 4));
     auto* hp = static_cast<uint32_t*>(hist.p);
-    for (int shift = 0; shift < bits; shift += 8) {
-        rs_hist<<<nblk, RS_T, 0, h->stream>>>(*keys, n, shift, hp, nblk);
+    for (int shift = 0; shift < std::max(bits, 1); shift += 8) {
+        rs_hist<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, seg_n, nbt, shift, hp);
         SIFT_HIP_TRY(h, hipGetLastError());
-        SIFT_TRY(exclusive_scan(h, hp, nblk * 256, levels, 0));
-        rs_scatter<<<nblk, RS_T, 0, h->stream>>>(*keys, *vals, *tk, *tv, n, shift, hp, nblk);
+        SIFT_TRY(exclusive_scan(h, hp, (uint32_t)(nblk * 256), levels, 0));
+        const uint32_t* vin = implicit_vals && shift == 0 ? nullptr : *vals;
+        rs_scatter<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, vin, *tk, *tv, seg_n, nbt, shift, hp);
         SIFT_HIP_TRY(h, hipGetLastError());
         std::swap(*keys, *tk);
         std::swap(*vals, *tv);
