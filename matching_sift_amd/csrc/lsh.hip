@@ -833,11 +833,8 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
 }
 
 // Sum over buckets of size^2 (= sum over sorted positions of the size of the
-// bucket holding them), one table per grid row: the thread at each bucket's
-// first position finds the bucket's end by a galloping search (a few loads
-// for small buckets), the rest only compare with their left neighbour.
-// Grid-stride; each block writes its partial sum to part[block] (the host
-// adds them: no atomics, the same total on every run).
+// bucket holding them).  Each block writes its partial sum to part[block]
+// (the host adds them: no atomics, the same total on every run).
 __device__ inline void block_sum_u64(unsigned long long s, unsigned long long* part) {
     __shared__ unsigned long long wsum[4];
 #pragma unroll
@@ -849,25 +846,52 @@ __device__ inline void block_sum_u64(unsigned long long s, unsigned long long* p
 
 __global__ __launch_bounds__(256) void bucket_sq(const uint32_t* __restrict__ keys, uint32_t n,
                                                  unsigned long long* __restrict__ part) {
+    // sum over sorted positions p of 2 (p - s(p)) + 1, s(p) the first position
+    // of p's bucket: over a bucket of size m that is m^2.  Block = 4096
+    // positions of one table (grid row), 16 consecutive ones per thread.
+    __shared__ long long wmax[4];
+    __shared__ uint32_t first_start;
     const uint32_t* tk = keys + (size_t)blockIdx.y * n;  // this table's sorted keys
-    unsigned long long s = 0;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t k = tk[p];
-        if (p == 0 || tk[p - 1] != k) {  // first position of its bucket
-            uint32_t lo = p, hi = n;  // tk[lo] == k
-            for (uint64_t d = 1;; d <<= 1) {
-                const uint64_t nx = (uint64_t)p + d;
-                if (nx >= n || tk[nx] != k) {
-                    hi = (uint32_t)min<uint64_t>(nx, n);
-                    break;
-                }
-                lo = (uint32_t)nx;
-            }
-            const unsigned long long len = upper_bound_u32(tk, lo + 1, hi, k) - p;
-            s += len * len;
+    const uint32_t cbase = blockIdx.x * 4096u, p0 = cbase + threadIdx.x * 16u;
+    uint32_t k[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) k[i] = p0 + i < n ? tk[p0 + i] : 0u;
+    const uint32_t prev = p0 > 0 && p0 - 1 < n ? tk[p0 - 1] : 0u;
+    // last bucket start at or before each position inside the thread (-1: before p0)
+    long long last = -1;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t p = p0 + i;
+        if (p < n && (p == 0 || k[i] != (i ? k[i - 1] : prev))) last = (long long)p;
+    }
+    // exclusive max over the block's threads of their last start
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    long long incl = last;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long y = __shfl_up(incl, o);
+        if (lane >= o) incl = max(incl, y);
+    }
+    if (lane == 63) wmax[w] = incl;
+    if (threadIdx.x == 0) {  // start of the bucket holding the chunk's first position
+        const uint32_t k0 = tk[cbase];
+        first_start = lower_bound_u32(tk, 0, cbase, k0);
+    }
+    __syncthreads();
+    long long carry = __shfl_up(incl, 1);
+    if (lane == 0) carry = -1;
+    for (int v = 0; v < w; v++) carry = max(carry, wmax[v]);
+    uint32_t s_run = carry >= 0 ? (uint32_t)carry : first_start;
+    unsigned long long sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t p = p0 + i;
+        if (p < n) {
+            if (p == 0 || k[i] != (i ? k[i - 1] : prev)) s_run = p;
+            sum += 2ull * (p - s_run) + 1ull;
         }
     }
-    block_sum_u64(s, part);
+    block_sum_u64(sum, part);
 }
 
 // The same sum from the bucket starts ([L][2^b + 1] positions).
@@ -935,7 +959,7 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
     if (ix->nbits <= OFFS_MAX_BITS)
         grid = dim3((unsigned)std::min<uint64_t>((((uint64_t)ix->L << ix->nbits) + 255) / 256, 256));
     else
-        grid = dim3((unsigned)std::min<size_t>((ix->n + 255) / 256, std::max(1, 1024 / ix->L)), (unsigned)ix->L);
+        grid = dim3((ix->n + 4095) / 4096, (unsigned)ix->L);
     const size_t nparts = (size_t)grid.x * grid.y;
     SIFT_TRY(reserve(h, ix->hist, nparts * 8));
     auto* part = static_cast<unsigned long long*>(ix->hist.p);
@@ -1012,41 +1036,60 @@ int pack_hash_launch(sift_context* h, sift_lsh_index* ix, const void* r, int typ
 }
 
 // flag: the handle's LSH flag words ([0] exactness, [1] scratch, [3] max |x|).
+// Sorts every table's (key, row) pairs (keys written, rows implicit) and
+// builds the bucket structures; the sorted arrays end in ix->keys / ix->vals.
+int sort_tables(sift_context* h, sift_lsh_index* ix) {
+    auto* keys = static_cast<uint32_t*>(ix->keys.p);
+    auto* vals = static_cast<uint32_t*>(ix->vals.p);
+    auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
+    auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
+    {  // each table sorted in its own segment; the row ids come from positions
+        ProfScope ps(h, PROF_LSH_SORT);
+        SIFT_TRY(radix_sort_pairs(h, &keys, &vals, &tk, &tv, ix->n, (uint32_t)ix->L, ix->nbits, ix->hist,
+                                  ix->levels, true));
+    }
+    if (keys != ix->keys.p) {
+        std::swap(ix->keys, ix->scratch_k);
+        std::swap(ix->vals, ix->scratch_v);
+    }
+    return build_offsets(h, ix);  // ends with the stream synchronized
+}
+
 int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, int* flag, bool force_float) {
     const uint32_t r_n = ix->n;
-    const int L = ix->L, nbits = ix->nbits;
     int rc = SIFT_OK;
     do {
         const uint32_t rows_pad = std::max<uint32_t>(32, ((r_n + 31) / 32) * 32);
-        const size_t total = (size_t)r_n * L;
+        const size_t total = (size_t)r_n * ix->L;
         for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
             if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
         if (rc != SIFT_OK) break;
         auto* keys = static_cast<uint32_t*>(ix->keys.p);
-        auto* vals = static_cast<uint32_t*>(ix->vals.p);
-        auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
-        auto* tv = static_cast<uint32_t*>(ix->scratch_v.p);
-        // hyperplanes over integer data: one pass writes the gather copy and the
-        // unsorted keys of every table
-        const bool fused = ix->hash_kind != SIFT_LSH_LEARNED;
-        bool hashed = false;
-        bool float_mode = force_float;
-        if (!float_mode && fused) {
+        if (!force_float && ix->hash_kind != SIFT_LSH_LEARNED) {
+            // hyperplanes: one pass writes the gather copy and the unsorted keys of
+            // every table; the sort follows without waiting for the exactness flag,
+            // and a base that is not all integers in [0,255] is built again in float
+            // mode
             if ((rc = reserve(h, ix->base_u8, std::max<size_t>((size_t)r_n * 128, 1))) != SIFT_OK) break;
             if ((rc = reserve(h, ix->base_nrm, std::max<size_t>((size_t)r_n * 4, 1))) != SIFT_OK) break;
             if (r_n) {
                 ProfScope ps(h, PROF_LSH_PROJ);
                 if ((rc = pack_hash_launch(h, ix, r, type, flag, keys)) != SIFT_OK) break;
             }
+            if ((rc = sort_tables(h, ix)) != SIFT_OK) break;
             int bad = 0;
             if (hipMemcpyAsync(&bad, flag, 4, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
                 hipStreamSynchronize(h->stream) != hipSuccess) {
                 rc = set_error(h, SIFT_ERR_HIP, "build flag");
                 break;
             }
-            float_mode = bad != 0;
-            hashed = !float_mode;
-        } else if (!float_mode) {  // learned hash: uint8 gather copy, exactness check, plain tiles
+            if (bad) return build_tables(h, ix, r, type, flag, true);
+            release(ix->scratch_k);
+            release(ix->scratch_v);
+            return SIFT_OK;
+        }
+        bool float_mode = force_float;
+        if (!float_mode) {  // learned hash: uint8 gather copy, exactness check, plain tiles
             {
                 ProfScope ps(h, PROF_PACK);
                 if ((rc = to_u8(h, r, type, r_n, ix->base_u8, ix->base_nrm, flag)) != SIFT_OK) break;
@@ -1085,25 +1128,11 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
             if ((rc = to_f32(h, r, type, r_n, ix->base_f32)) != SIFT_OK) break;
             if ((rc = pack_quant(h, r, type, r_n, rows_pad, ix->qexp, h->b_set, flag + 1)) != SIFT_OK) break;
         }
-        if (!hashed) {
-            {
-                ProfScope ps(h, PROF_LSH_PROJ);
-                if ((rc = hash_codes(h, ix, h->b_set, r_n, keys, r_n, true)) != SIFT_OK) break;
-            }
+        {
+            ProfScope ps(h, PROF_LSH_PROJ);
+            if ((rc = hash_codes(h, ix, h->b_set, r_n, keys, r_n, true)) != SIFT_OK) break;
         }
-        {  // each table sorted in its own segment; the row ids come from positions
-            ProfScope ps(h, PROF_LSH_SORT);
-            if ((rc = radix_sort_pairs(h, &keys, &vals, &tk, &tv, r_n, (uint32_t)L, nbits, ix->hist, ix->levels,
-                                       true)) != SIFT_OK)
-                break;
-        }
-        // keep the sorted arrays in ix->keys / ix->vals
-        if (keys != ix->keys.p) {
-            std::swap(ix->keys, ix->scratch_k);
-            std::swap(ix->vals, ix->scratch_v);
-        }
-        if ((rc = build_offsets(h, ix)) != SIFT_OK) break;
-        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = set_error(h, SIFT_ERR_HIP, "sync"); break; }
+        if ((rc = sort_tables(h, ix)) != SIFT_OK) break;
         release(ix->scratch_k);
         release(ix->scratch_v);
     } while (0);

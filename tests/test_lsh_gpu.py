@@ -333,6 +333,22 @@ def test_lsh_bucket_stat(handle, nbits, L):
     ix.close()
 
 
+@pytest.mark.parametrize("nbits", [12, 24])
+@pytest.mark.parametrize("distinct", [1, 40])
+def test_lsh_bucket_stat_large_buckets(handle, nbits, distinct):
+    """Buckets far larger than the statistic's 4096-position blocks (rows
+    repeated from a few distinct ones, or one row only)."""
+    src = datagen.base(distinct, seed=52)
+    b = np.ascontiguousarray(src[np.random.default_rng(5).integers(0, distinct, 13_000)])
+    ix = LSHIndex(handle, b, nbits, 2, seed=17)
+    say("lsh bucket stat, large buckets", nbits, distinct)
+    cb = ix.codes(b)
+    want = sum(float((np.unique(cb[t], return_counts=True)[1].astype(np.float64) ** 2).sum())
+               for t in range(2)) / (2 * len(b))
+    assert ix.bucket_stat() == pytest.approx(want, rel=1e-12)
+    ix.close()
+
+
 @pytest.mark.parametrize("seed", [3, 17])
 def test_lsh_random_configs(handle, monkeypatch, seed):
     """Random (nbits, L, probes, hyperplane kind) with base and query sizes at
