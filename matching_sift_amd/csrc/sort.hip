@@ -15,12 +15,13 @@
 namespace sift {
 namespace {
 
-constexpr int RS_T = 256;            // threads per block
+constexpr int RS_T = 512;            // threads per block (at least one per digit)
 constexpr int RS_I = 16;             // keys per thread
 constexpr int RS_CHUNK = RS_T * RS_I;  // keys per block
 constexpr int RS_W = RS_T / 64;      // waves per block
 constexpr int RS_SUB = RS_CHUNK / RS_W;  // keys per wave, walked in order
 constexpr int RS_STEPS = RS_SUB / 64;  // keys per lane
+static_assert(RS_T >= 256, "one thread per digit");
 
 // The pairs form nseg segments of seg_n consecutive elements, each sorted on
 // its own (segment s keeps positions [s seg_n, (s + 1) seg_n)); blocks never
@@ -46,7 +47,7 @@ __device__ inline SegChunk seg_chunk(uint32_t blk, uint32_t seg_n, uint32_t nbt)
 __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t seg_n, uint32_t nbt,
                                                 int shift, uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
     __syncthreads();
     const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
 #pragma unroll
@@ -55,7 +56,7 @@ __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ key
         if (idx < c.m) atomicAdd(&cnt[(keys[c.base + idx] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[c.hrow + (size_t)threadIdx.x * nbt] = cnt[threadIdx.x];
+    if (threadIdx.x < 256) hist[c.hrow + (size_t)threadIdx.x * nbt] = cnt[threadIdx.x];
 }
 
 // Stable scatter of one chunk: ranks come from wave-wide ballots, the chunk
@@ -70,10 +71,9 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
                                                    int shift, const uint32_t* __restrict__ offs) {
     __shared__ uint32_t sk[RS_CHUNK], sv[RS_CHUNK];
     __shared__ uint32_t wh[RS_W][256];  // per-wave digit counts, then local cursors
-    __shared__ uint32_t lstart[256], gstart[256], wsum[RS_W];
+    __shared__ uint32_t delta[256], wsum[4];  // per digit: output position - position in the chunk
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < RS_W; k++) wh[k][threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < RS_W * 256; i += RS_T) (&wh[0][0])[i] = 0;
     __syncthreads();
     const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
     const uint32_t sub = (uint32_t)w * RS_SUB;  // the wave's part of the chunk
@@ -87,25 +87,27 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
     for (int step = 0; step < RS_STEPS; step++)
         if (sub + (uint32_t)step * 64 + lane < c.m) atomicAdd(&wh[w][(key[step] >> shift) & 255u], 1u);
     __syncthreads();
-    {  // per digit: start of its run in the chunk (scan over digits), this
-       // block's global offset, and each wave's local cursor
-        const int bin = threadIdx.x;
-        uint32_t tot = 0;
+    // per digit (threads 0..255, waves 0..3): start of its run in the chunk
+    // (scan over digits), the block's output offset, and each wave's local cursor
+    const int bin = threadIdx.x;
+    uint32_t tot = 0, x = 0;
+    if (bin < 256) {
 #pragma unroll
         for (int k = 0; k < RS_W; k++) tot += wh[k][bin];
-        uint32_t x = tot;
+        x = tot;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
             const uint32_t y = __shfl_up(x, o);
             if (lane >= o) x += y;
         }
         if (lane == 63) wsum[w] = x;
-        __syncthreads();
+    }
+    __syncthreads();
+    if (bin < 256) {
         uint32_t pre = 0;
         for (int k = 0; k < w; k++) pre += wsum[k];
         const uint32_t ls = pre + x - tot;
-        lstart[bin] = ls;
-        gstart[bin] = offs[c.hrow + (size_t)bin * nbt];
+        delta[bin] = offs[c.hrow + (size_t)bin * nbt] - ls;  // modulo 2^32
         uint32_t run = ls;
 #pragma unroll
         for (int k = 0; k < RS_W; k++) {
@@ -142,8 +144,7 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < c.m; i += RS_T) {
         const uint32_t k = sk[i];
-        const uint32_t d = (k >> shift) & 255u;
-        const uint32_t pos = gstart[d] + (i - lstart[d]);
+        const uint32_t pos = i + delta[(k >> shift) & 255u];
         kout[pos] = k;
         vout[pos] = sv[i];
     }
