@@ -181,7 +181,7 @@ struct FusedArgs {
     float* out_dist;
     uint32_t* out_matches;
     float ratio;
-    int neg64;  // pairs kernel, I8: 64, passed at run time so a composite takes one v_mad_i32_i24
+    int comp_mul;  // pairs kernel, I8: 128, passed at run time so a composite takes one v_mad_i32_i24
     uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
     // 8-wave blocks: polls allowed per stage hand-off wait, and the status word
     // a wave ORs 1 into when a wait runs out of them (its results are then void)

@@ -224,10 +224,10 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     key_t thr_h[QW];
     uint32_t i1[QW], i2[QW];
     // Packed mode (pairs kernel, integer kinds): each lane keeps its top-2 as
-    // int32 "composites" key*32 + slot so one v_min + one v_med3 per row
+    // int32 "composites" key*64 + slot so one v_min + one v_med3 per row
     // maintain it with no branch (see the insert below).
     constexpr bool COMP = PAIRS && KIND != KIND_F32;
-    constexpr int CNONE = 0x7FFFFFE0;  // above every real composite, low bits 0
+    constexpr int CNONE = 0x7FFFFFC0;  // above every real composite, low bits 0
     int c1[QW], c2[QW];
     const uint32_t lq = (uint32_t)(lane & 31);
 #pragma unroll
@@ -381,48 +381,46 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 
         if constexpr (COMP) {
             // Every row is inserted, branch-free.  A row's composite is
-            // key*32 + 16 + j (j = accumulator register 0..15, increasing
-            // with the row within the lane's half of the tile); entries kept
-            // from earlier tiles are stored as key*32 (slot bits cleared).
+            // key*64 + 32 + r (r = the row within its 32-row tile); entries
+            // kept from earlier tiles are stored as key*64 (slot bits cleared).
             // Order by composite = order by (key, row): a new row loses a tie
             // against a kept one (earlier tile, lower index) and wins or loses
-            // within its tile by j.  Keys are exact integers below 2^25 in
-            // magnitude for SIFT data, so key*32 + 31 fits in int32.
+            // within its tile by r.  Keys are exact integers below 2^24 in
+            // magnitude for SIFT data, so key*64 + 63 fits in int32.  I8: the
+            // packed base norms arrive as (c << 6) | (32 + r) (compose_slots),
+            // so a composite is one v_mad_i32_i24: 128 q''.b' + that.
 #pragma unroll
             for (int tt = 0; tt < TPS; tt++) {
                 if (tt >= ntile) break;  // uniform: slab end
                 acc_t acc[QW];
                 tile_acc(tt, acc);
-                int w[16];  // I8: c * 32 + 16 + j, so composite = w + 64 q''.b' (one v_mad_i32_i24)
-                if constexpr (!FLOAT_KEY) {
-#pragma unroll
-                    for (int j = 0; j < 16; j++) w[j] = (nlr[tt][j] << 5) | (16 + j);
-                }
                 // base pad rows carry PAD_KEY, so they need no mask here: they
                 // lose to every real row and are dropped after the loop
-                const uint32_t rowbase = (uint32_t)(ts + tt) * 32u + 4u * h;
+                const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) {
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         int cv;
                         if constexpr (FLOAT_KEY)
-                            cv = ((int)acc[qb][j] << 5) | (16 + j);
+                            cv = ((int)acc[qb][j] << 6) | (32 + 8 * (j >> 2) + 4 * h + (j & 3));
                         else
-                            cv = __mul24((int)acc[qb][j], a.neg64) + w[j];
+                            cv = __mul24((int)acc[qb][j], a.comp_mul) + nlr[tt][j];
+                        // c1 <= c2 always, so the new 2nd smallest of the three
+                        // is their median
                         const int lo = min(c1[qb], c2[qb]), hi = max(c1[qb], c2[qb]);
-                        c2[qb] = max(lo, min(hi, cv));  // v_med3_i32: 2nd smallest of three
+                        c2[qb] = max(lo, min(hi, cv));  // v_med3_i32
                         c1[qb] = min(c1[qb], cv);
                     }
                     // new entries carry slot bits: record their rows, then clear
-                    const bool n1 = (c1[qb] & 16) != 0, n2 = (c2[qb] & 16) != 0;
-                    const uint32_t r1 = rowbase + (uint32_t)(c1[qb] & 3) + ((uint32_t)(c1[qb] & 12) << 1);
-                    const uint32_t r2 = rowbase + (uint32_t)(c2[qb] & 3) + ((uint32_t)(c2[qb] & 12) << 1);
+                    const bool n1 = (c1[qb] & 32) != 0, n2 = (c2[qb] & 32) != 0;
+                    const uint32_t r1 = tile_row0 + (uint32_t)(c1[qb] & 31);
+                    const uint32_t r2 = tile_row0 + (uint32_t)(c2[qb] & 31);
                     // a kept 2nd entry is the old 1st if the 1st was displaced
                     i2[qb] = n2 ? r2 : (n1 ? i1[qb] : i2[qb]);
                     i1[qb] = n1 ? r1 : i1[qb];
-                    c1[qb] &= ~31;
-                    c2[qb] &= ~31;
+                    c1[qb] &= ~63;
+                    c2[qb] &= ~63;
                 }
             }
         } else {
@@ -551,8 +549,8 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         for (int qb = 0; qb < QW; qb++) {
             if (i1[qb] >= task.r_lim) i1[qb] = 0xFFFFFFFFu;
             if (i2[qb] >= task.r_lim) i2[qb] = 0xFFFFFFFFu;
-            m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> 5) : Key<KIND>::none();
-            m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> 5) : Key<KIND>::none();
+            m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> 6) : Key<KIND>::none();
+            m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> 6) : Key<KIND>::none();
         }
     }
     if constexpr (PAIRS && KIND == KIND_FP8) {

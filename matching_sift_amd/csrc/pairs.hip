@@ -68,6 +68,17 @@ constexpr int FP8_SLOTS = 4;
 // (uint8 rows, int32 |q - b|^2 = |q|^2 + |b|^2 - 2 q.b with v_dot4_u32_u8),
 // then a butterfly over the 16 lanes keeps the two smallest (d, index) pairs,
 // packed as d * 2^32 + index so one integer order is the lexicographic one.
+// I8 base norms of the pairs kernel, in accumulator-lane order: c becomes
+// (c << 6) | (32 + r), r the row within its 32-row tile, so the kernel forms
+// each row's composite key*64 + 32 + r with a single multiply-add.
+__global__ void compose_slots(int* __restrict__ nl, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = i & 31, hl = p >> 4, j = p & 15;
+    const uint32_t r = 8 * (j >> 2) + 4 * hl + (j & 3);
+    nl[i] = (int)(((uint32_t)nl[i] << 6) | (32u + r));
+}
+
 __global__ __launch_bounds__(256) void fp8_rerank(const uint32_t* __restrict__ cand,
                                                   const uint4* __restrict__ q_u8, const int* __restrict__ q_n2,
                                                   const uint4* __restrict__ r_u8, const int* __restrict__ r_n2,
@@ -216,7 +227,7 @@ FusedArgs make_args(sift_pairs_plan* p, uint32_t* idx, float* dist, uint32_t* ma
     a.out_dist = dist;
     a.out_matches = matches;
     a.ratio = ratio;
-    a.neg64 = 64;  // I8 composite = 64 q''.b' + (c << 5 | slot) = key * 32 + slot
+    a.comp_mul = 128;  // I8 composite = 128 q''.b' + ((c << 6) | 32 + r) = key * 64 + 32 + r
     a.out_cand = static_cast<uint32_t*>(p->cand.p);
     a.wait_polls = p->h->wait_polls;
     a.status = wait_status(p->h);
@@ -234,6 +245,11 @@ int run_sequence(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx
                              PACK_QUERY, p->qs, flag));
         SIFT_TRY(pack_launch(h, r, p->type, p->r_rows, static_cast<const int32_t*>(p->rmap.p), p->kind,
                              PACK_BASE, p->bs, flag));
+        if (p->kind == KIND_I8 && p->bs.rows_pad) {
+            compose_slots<<<(p->bs.rows_pad + 255) / 256, 256, 0, h->stream>>>(static_cast<int*>(p->bs.norm_lane.p),
+                                                                                p->bs.rows_pad);
+            SIFT_HIP_TRY(h, hipGetLastError());
+        }
     }
     if (p->kind == KIND_FP8) {
         ProfScope ps(h, prof ? PROF_PACK : -1);
