@@ -188,12 +188,17 @@ class BruteForceIndex:
             outputs = (DeviceArray.empty((M, 2), np.uint32), DeviceArray.empty((M, 2), np.float32),
                        None, None)
         idx, dist = outputs[0], outputs[1]
+        srt = outputs[2] if len(outputs) > 2 else None
+        mat = outputs[3] if len(outputs) > 3 else None
         check(lib().sift_bf_index_query(self.handle.h, self.ix, qd.ptr, _input_type(qd, type_), M,
                                         idx.ptr if idx else None, dist.ptr if dist else None,
-                                        None, None, float(ratio), 1 if blocking else 0),
+                                        srt.ptr if srt else None, mat.ptr if mat else None, float(ratio),
+                                        1 if blocking else 0),
               self.handle.h)
         if not blocking:
             return outputs
+        if mat is not None:
+            return idx.numpy(), dist.numpy(), mat.numpy()
         return idx.numpy(), dist.numpy()
 
     def close(self):

@@ -68,7 +68,7 @@ int check_wait(sift_context* h, const char* what) {
 
 const char* prof_name(int k) {
     static const char* names[PROF_COUNT] = {"pack", "fused", "merge", "seed", "lsh_proj", "lsh_sort",
-                                            "lsh_query", "fp8", "pairs", "mlp"};
+                                            "lsh_query", "fp8", "pairs", "mlp", "lsh_aux"};
     return (k >= 0 && k < PROF_COUNT) ? names[k] : "?";
 }
 

@@ -45,7 +45,7 @@ int wait_error(sift_context* h, int status, const char* what);
 
 // ---------------------------------------------------------- profiling ----
 enum ProfKernel { PROF_PACK = 0, PROF_FUSED, PROF_MERGE, PROF_SEED, PROF_LSH_PROJ, PROF_LSH_SORT,
-                  PROF_LSH_QUERY, PROF_FP8, PROF_PAIRS, PROF_MLP, PROF_COUNT };
+                  PROF_LSH_QUERY, PROF_FP8, PROF_PAIRS, PROF_MLP, PROF_LSH_AUX, PROF_COUNT };
 const char* prof_name(int k);
 // RAII pair of events around one launch on the handle's stream.
 struct ProfScope {

@@ -1251,7 +1251,10 @@ int sift_lsh_query(sift_handle_t h, sift_lsh_index_t ix, const void* q, int type
     SIFT_TRY(reserve(h, sc->flag, 16));
     int* flag = static_cast<int*>(sc->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, 0, 8, h->stream));
-    if (!ix->float_mode) SIFT_TRY(to_u8(h, q, type, q_n, sc->q_u8, sc->q_nrm, flag));
+    if (!ix->float_mode) {
+        ProfScope ps(h, PROF_LSH_AUX);  // the queries' uint8 copy
+        SIFT_TRY(to_u8(h, q, type, q_n, sc->q_u8, sc->q_nrm, flag));
+    }
     SIFT_TRY(pack_for_hash(h, ix, q, type, q_n, sc, flag));
     const uint32_t* qcodes = nullptr;
     {
@@ -1400,12 +1403,15 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     SIFT_TRY(reserve(h, sc->fb_map, (size_t)query_rows_pad(bf->kind, q_n) * 4));
     auto* map = static_cast<int32_t*>(sc->fb_map.p);
     const unsigned gq = (q_n + 255) / 256;
-    fallback_flags<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos);
-    SIFT_HIP_TRY(h, hipGetLastError());
-    SIFT_TRY(exclusive_scan(h, pos, q_n, sc->ulevels, 0));
-    fallback_compact<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos, map,
-                                                static_cast<uint32_t*>(sc->fb_count.p));
-    SIFT_HIP_TRY(h, hipGetLastError());
+    {
+        ProfScope ps(h, PROF_LSH_AUX);  // the list of queries for the exact path
+        fallback_flags<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos);
+        SIFT_HIP_TRY(h, hipGetLastError());
+        SIFT_TRY(exclusive_scan(h, pos, q_n, sc->ulevels, 0));
+        fallback_compact<<<gq, 256, 0, h->stream>>>(nc, dq, q_n, min_cand, accept_ratio, pos, map,
+                                                    static_cast<uint32_t*>(sc->fb_count.p));
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     uint32_t cnt = 0;
     SIFT_HIP_TRY(h, hipMemcpyAsync(&cnt, sc->fb_count.p, 4, hipMemcpyDeviceToHost, h->stream));
     SIFT_HIP_TRY(h, hipStreamSynchronize(h->stream));
@@ -1415,6 +1421,7 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     if (cnt == 0) return SIFT_OK;
     const uint32_t rows_pad = query_rows_pad(bf->kind, cnt);
     if (rows_pad > cnt) {
+        ProfScope ps(h, PROF_LSH_AUX);
         map_tail<<<(rows_pad - cnt + 255) / 256, 256, 0, h->stream>>>(map, cnt, rows_pad);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
@@ -1429,7 +1436,10 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
     int* flag = static_cast<int*>(sc->flag.p);
     SIFT_HIP_TRY(h, hipMemsetAsync(flag, bf->coarse ? 1 : 0, 4, h->stream));
     const auto* fb_map = static_cast<const int32_t*>(sc->fb_map.p);
-    SIFT_TRY(pack_launch(h, q, type, 0, fb_map, bf->kind, PACK_QUERY, qs, flag));
+    {
+        ProfScope ps(h, PROF_PACK);
+        SIFT_TRY(pack_launch(h, q, type, 0, fb_map, bf->kind, PACK_QUERY, qs, flag));
+    }
     SIFT_TRY(reserve(h, sc->fb_out, (size_t)cnt * 20));
     auto* oi = static_cast<uint32_t*>(sc->fb_out.p);
     auto* od = reinterpret_cast<float*>(oi + 2 * (size_t)cnt);
@@ -1443,8 +1453,11 @@ int sift_lsh_query_fallback(sift_handle_t h, sift_lsh_index_t ix, sift_bf_index_
                          fb_map};
     }
     SIFT_TRY(knn2_packed(h, qs, bf->base, cnt, bf->r_n, o));
-    scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(fb_map, cnt, oi, od, om, idx, dist, matches);
-    SIFT_HIP_TRY(h, hipGetLastError());
+    {
+        ProfScope ps(h, PROF_LSH_AUX);
+        scatter_exact<<<(cnt + 255) / 256, 256, 0, h->stream>>>(fb_map, cnt, oi, od, om, idx, dist, matches);
+        SIFT_HIP_TRY(h, hipGetLastError());
+    }
     SIFT_TRY(check_wait(h, "sift_lsh_query_fallback"));
     if (bf->kind == KIND_I8 || bf->kind == KIND_FP8) {  // those kinds cannot hold other data
         int bad = 0;
