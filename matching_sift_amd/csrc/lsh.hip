@@ -34,6 +34,9 @@
 
 namespace sift {
 int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
+int suffix_min(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
+int scan_reserve(sift_context* h, uint32_t n, DevBuf* levels);
+int sort_reserve(sift_context* h, uint32_t seg_n, uint32_t nseg, DevBuf& hist, DevBuf* levels);
 int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk, uint32_t** tv,
                      uint32_t seg_n, uint32_t nseg, int bits, DevBuf& hist, DevBuf* levels, bool implicit_vals);
 }  // namespace sift
@@ -813,6 +816,9 @@ int check_flag(sift_context* h, int* flag, const char* what) {
 // sorted range [t*n, (t+1)*n) whose key is >= (t << b | c); offs[t][2^b] is
 // the table's end.  One thread per (table, code), each a binary search.
 constexpr int OFFS_MAX_BITS = 20;
+constexpr uint32_t OFFS_SEARCH_MAX = 1u << 20;  // entries up to which bucket starts are searched
+// Bucket starts [L][2^b + 1] of a small table set: one thread per entry, a
+// binary search of its code in its table's sorted keys.
 __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, int b, uint32_t L,
                                uint32_t* __restrict__ offs) {
     const uint32_t per = (1u << b) + 1u;
@@ -830,6 +836,26 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
         }
     }
     offs[i] = lo;
+}
+
+// The same for large table sets, from the sorted keys: offs_init sets every
+// entry past the end (all ones) and each table's last entry to its end
+// (t + 1) n; offs_mark writes the first position of every present code;
+// a suffix minimum then gives each absent code the start of the next present
+// one (an empty bucket).  Tables lie in order with increasing positions, so
+// one suffix minimum over the whole array never crosses into another table.
+__global__ void offs_init(uint32_t* __restrict__ offs, uint32_t per, uint32_t total, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < total) offs[i] = (i % per == per - 1) ? (i / per + 1) * n : 0xFFFFFFFFu;
+}
+
+__global__ void offs_mark(const uint32_t* __restrict__ keys, uint32_t n, uint32_t mask, uint32_t per,
+                          uint32_t* __restrict__ offs) {
+    const uint32_t t = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* tk = keys + (size_t)t * n;
+    const uint32_t k = tk[i];
+    if (i == 0 || tk[i - 1] != k) offs[(size_t)t * per + (k & mask)] = t * n + i;
 }
 
 // Sum over buckets of size^2 (= sum over sorted positions of the size of the
@@ -955,25 +981,40 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
     ix->cand_per_lookup = 0.0;
     if (ix->n == 0) return SIFT_OK;
     const size_t total = (size_t)ix->L * ix->n;
+    const bool direct = ix->nbits <= OFFS_MAX_BITS;
+    const uint32_t per = direct ? (1u << ix->nbits) + 1u : 0u;
+    const uint32_t no = (uint32_t)ix->L * per;
     dim3 grid;
-    if (ix->nbits <= OFFS_MAX_BITS)
-        grid = dim3((unsigned)std::min<uint64_t>((((uint64_t)ix->L << ix->nbits) + 255) / 256, 256));
+    if (direct)
+        grid = dim3((unsigned)std::min<uint64_t>((((uint64_t)ix->L << ix->nbits) + 255) / 256, 4096));
     else
         grid = dim3((ix->n + 4095) / 4096, (unsigned)ix->L);
     const size_t nparts = (size_t)grid.x * grid.y;
+    // every buffer first: an allocation between launches would stall the queue
     SIFT_TRY(reserve(h, ix->hist, nparts * 8));
+    if (direct) {
+        SIFT_TRY(reserve(h, ix->offs, (size_t)no * 4));
+        SIFT_TRY(scan_reserve(h, no, ix->levels));
+    }
     auto* part = static_cast<unsigned long long*>(ix->hist.p);
     {
         ProfScope ps(h, PROF_LSH_SORT);  // bucket structures count with the sort
-        if (ix->nbits <= OFFS_MAX_BITS) {
-            const size_t no = (size_t)ix->L * ((1u << ix->nbits) + 1u);
-            SIFT_TRY(reserve(h, ix->offs, no * 4));
-            bucket_offsets<<<(unsigned)((no + 255) / 256), 256, 0, h->stream>>>(
-                static_cast<const uint32_t*>(ix->keys.p), ix->n, ix->nbits, (uint32_t)ix->L,
-                static_cast<uint32_t*>(ix->offs.p));
-            SIFT_HIP_TRY(h, hipGetLastError());
-            bucket_sq_offs<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->offs.p), ix->nbits,
-                                                        (uint32_t)ix->L, part);
+        if (direct) {
+            auto* offs = static_cast<uint32_t*>(ix->offs.p);
+            const auto* keys = static_cast<const uint32_t*>(ix->keys.p);
+            if (no <= OFFS_SEARCH_MAX) {  // few entries: a search each beats three passes over the keys
+                bucket_offsets<<<(no + 255) / 256, 256, 0, h->stream>>>(keys, ix->n, ix->nbits, (uint32_t)ix->L,
+                                                                          offs);
+                SIFT_HIP_TRY(h, hipGetLastError());
+            } else {
+                offs_init<<<(no + 255) / 256, 256, 0, h->stream>>>(offs, per, no, ix->n);
+                SIFT_HIP_TRY(h, hipGetLastError());
+                offs_mark<<<dim3((ix->n + 255) / 256, (unsigned)ix->L), 256, 0, h->stream>>>(
+                    keys, ix->n, (1u << ix->nbits) - 1u, per, offs);
+                SIFT_HIP_TRY(h, hipGetLastError());
+                SIFT_TRY(suffix_min(h, offs, no, ix->levels, 0));
+            }
+            bucket_sq_offs<<<grid, 256, 0, h->stream>>>(offs, ix->nbits, (uint32_t)ix->L, part);
         } else {
             bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, part);
         }
@@ -1039,6 +1080,7 @@ int pack_hash_launch(sift_context* h, sift_lsh_index* ix, const void* r, int typ
 // Sorts every table's (key, row) pairs (keys written, rows implicit) and
 // builds the bucket structures; the sorted arrays end in ix->keys / ix->vals.
 int sort_tables(sift_context* h, sift_lsh_index* ix) {
+    SIFT_TRY(sort_reserve(h, ix->n, (uint32_t)ix->L, ix->hist, ix->levels));
     auto* keys = static_cast<uint32_t*>(ix->keys.p);
     auto* vals = static_cast<uint32_t*>(ix->vals.p);
     auto* tk = static_cast<uint32_t*>(ix->scratch_k.p);
@@ -1064,6 +1106,13 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
         for (DevBuf* b : {&ix->keys, &ix->vals, &ix->scratch_k, &ix->scratch_v})
             if ((rc = reserve(h, *b, std::max<size_t>(total, 1) * 4)) != SIFT_OK) break;
         if (rc != SIFT_OK) break;
+        // the sort's and the bucket starts' scratch too, before the first launch
+        if ((rc = sort_reserve(h, r_n, (uint32_t)ix->L, ix->hist, ix->levels)) != SIFT_OK) break;
+        if (r_n && ix->nbits <= OFFS_MAX_BITS) {
+            const uint32_t no = (uint32_t)ix->L * ((1u << ix->nbits) + 1u);
+            if ((rc = reserve(h, ix->offs, (size_t)no * 4)) != SIFT_OK) break;
+            if ((rc = scan_reserve(h, no, ix->levels)) != SIFT_OK) break;
+        }
         auto* keys = static_cast<uint32_t*>(ix->keys.p);
         if (!force_float && ix->hash_kind != SIFT_LSH_LEARNED) {
             // hyperplanes: one pass writes the gather copy and the unsorted keys of
