@@ -45,7 +45,7 @@ __device__ inline SegChunk seg_chunk(uint32_t blk, uint32_t seg_n, uint32_t nbt)
 }
 
 __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t seg_n, uint32_t nbt,
-                                                int shift, uint32_t* __restrict__ hist) {
+                                                int shift, uint32_t dmask, uint32_t* __restrict__ hist) {
     __shared__ uint32_t cnt[256];
     if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ key
 #pragma unroll
     for (int i = 0; i < RS_I; i++) {
         const uint32_t idx = (uint32_t)i * RS_T + threadIdx.x;
-        if (idx < c.m) atomicAdd(&cnt[(keys[c.base + idx] >> shift) & 255u], 1u);
+        if (idx < c.m) atomicAdd(&cnt[(keys[c.base + idx] >> shift) & dmask], 1u);
     }
     __syncthreads();
     if (threadIdx.x < 256) hist[c.hrow + (size_t)threadIdx.x * nbt] = cnt[threadIdx.x];
@@ -68,7 +68,8 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout, uint32_t seg_n, uint32_t nbt,
-                                                   int shift, const uint32_t* __restrict__ offs) {
+                                                   int shift, int dbits, const uint32_t* __restrict__ offs) {
+    const uint32_t dmask = (1u << dbits) - 1u;  // the pass's significant digit bits
     __shared__ uint32_t sk[RS_CHUNK], sv[RS_CHUNK];
     __shared__ uint32_t wh[RS_W][256];  // per-wave digit counts, then local cursors
     __shared__ uint32_t delta[256], wsum[4];  // per digit: output position - position in the chunk
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
     }
 #pragma unroll
     for (int step = 0; step < RS_STEPS; step++)
-        if (sub + (uint32_t)step * 64 + lane < c.m) atomicAdd(&wh[w][(key[step] >> shift) & 255u], 1u);
+        if (sub + (uint32_t)step * 64 + lane < c.m) atomicAdd(&wh[w][(key[step] >> shift) & dmask], 1u);
     __syncthreads();
     // per digit (threads 0..255, waves 0..3): start of its run in the chunk
     // (scan over digits), the block's output offset, and each wave's local cursor
@@ -124,14 +125,24 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
         const uint32_t idx = sub + (uint32_t)step * 64 + lane;
         const bool valid = idx < c.m;
         const uint32_t val = !valid ? 0u : vin ? vin[c.base + idx] : vbase + idx;
-        const uint32_t d = (key[step] >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+        const uint32_t d = (key[step] >> shift) & dmask;
+        // lanes with the same digit: for each significant bit, the lanes that
+        // agree with this lane on it.  s = 0 where this lane's bit is set, all
+        // ones where it is clear, so (ballot of set bits) ^ s selects either
+        // that ballot or its complement: one bit-field extract, one compare
+        // and four 32-bit logic ops per bit
+        const uint64_t vb = __builtin_amdgcn_ballot_w64(valid);
+        uint32_t p_lo = (uint32_t)vb, p_hi = (uint32_t)(vb >> 32);
+        const uint32_t nd = ~d;
 #pragma unroll
         for (int bit = 0; bit < 8; bit++) {
-            const bool set = (d >> bit) & 1u;
-            const uint64_t bb = __ballot(valid && set);
-            peers &= set ? bb : ~bb;
+            if (bit >= dbits) break;  // uniform
+            const uint32_t sx = (uint32_t)((int)(nd << (31 - bit)) >> 31);
+            const uint64_t bb = __builtin_amdgcn_ballot_w64(sx == 0u) & vb;
+            p_lo &= (uint32_t)bb ^ sx;
+            p_hi &= (uint32_t)(bb >> 32) ^ sx;
         }
+        const uint64_t peers = ((uint64_t)p_hi << 32) | p_lo;
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
         const uint32_t pos = wh[w][d];
         if (valid) {
@@ -144,7 +155,7 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < c.m; i += RS_T) {
         const uint32_t k = sk[i];
-        const uint32_t pos = i + delta[(k >> shift) & 255u];
+        const uint32_t pos = i + delta[(k >> shift) & dmask];
         kout[pos] = k;
         vout[pos] = sv[i];
     }
@@ -337,11 +348,13 @@ int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t
     SIFT_TRY(reserve(h, hist, (size_t)nblk * 256 * 4));
     auto* hp = static_cast<uint32_t*>(hist.p);
     for (int shift = 0; shift < std::max(bits, 1); shift += 8) {
-        rs_hist<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, seg_n, nbt, shift, hp);
+        // digit bits at or above `bits` are constant within a segment: masked off
+        const int dbits = std::min(8, std::max(bits, 1) - shift);
+        rs_hist<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, seg_n, nbt, shift, (1u << dbits) - 1u, hp);
         SIFT_HIP_TRY(h, hipGetLastError());
         SIFT_TRY(exclusive_scan(h, hp, (uint32_t)(nblk * 256), levels, 0));
         const uint32_t* vin = implicit_vals && shift == 0 ? nullptr : *vals;
-        rs_scatter<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, vin, *tk, *tv, seg_n, nbt, shift, hp);
+        rs_scatter<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, vin, *tk, *tv, seg_n, nbt, shift, dbits, hp);
         SIFT_HIP_TRY(h, hipGetLastError());
         std::swap(*keys, *tk);
         std::swap(*vals, *tv);
