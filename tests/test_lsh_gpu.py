@@ -53,7 +53,7 @@ def check_index(ix, q, b, ratio=0.8):
     return idx, nc
 
 
-@pytest.mark.parametrize("nbits,L", [(1, 1), (8, 2), (12, 4), (16, 8), (20, 3), (24, 2), (32, 1)])
+@pytest.mark.parametrize("nbits,L", [(1, 1), (8, 2), (12, 4), (16, 8), (20, 3), (24, 2), (32, 1), (8, 12), (4, 20)])
 def test_lsh_matches_oracle(handle, nbits, L):
     say("lsh", nbits, L)
     b = datagen.base(12_000, seed=31)
