@@ -157,6 +157,32 @@ def test_pairs_config3_sample(handle):
     plan.close()
 
 
+@pytest.mark.parametrize("compute", [COMPUTE_I8, COMPUTE_BF16])
+def test_pairs_config3_full(handle, compute):
+    """All 512 pairs of the batched-pairs workload (1.21 M queries), autotuned
+    and graph-replayed: every pair equals the brute-force kernel on that pair
+    alone (itself bit-exact against the oracle), and the first 16 pairs equal
+    the oracle's host_brute directly."""
+    q, q_off, r, r_off, pairs = datagen.config3()
+    say("config3 full", compute, q.shape, r.shape)
+    qd, rd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(r)
+    plan = PairsPlan(handle, q_off, r_off, pairs, compute=compute)
+    outs = plan.outputs()
+    got = None
+    for rep in range(2):  # the first execute autotunes and captures the graph
+        got = plan.execute(qd, rd, outputs=outs)
+    gi, gd, gm = got
+    n16 = int(q_off[16])
+    assert_same((gi[:n16], gd[:n16], gm[:n16]), expected_pairs(q, q_off, r, r_off, pairs[:16]), "oracle")
+    say("per-pair brute force on 512 pairs")
+    for p in range(len(pairs)):
+        a, b = int(q_off[p]), int(q_off[p + 1])
+        ei, ed, _, em = knn2(handle, q[a:b], r[r_off[p]:r_off[p + 1]], compute=compute, ratio=0.8)
+        assert_same((gi[a:b], gd[a:b], gm[a:b]), (ei, ed, em), "pair %d" % p)
+    assert (gm != UMAX).mean() > 0.2
+    plan.close()
+
+
 @pytest.mark.parametrize("seed", [5, 9, 13])
 def test_pairs_random_batches(handle, seed):
     """Random batches: 1-5 images a side with sizes at tile and group edges
