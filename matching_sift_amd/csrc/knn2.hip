@@ -337,29 +337,57 @@ void choose_slabs(int n_tiles, int n_groups, int cap, int tps, int* n_slabs, int
     *tps_slab = std::max(tps, ((n_tiles + best_ns - 1) / best_ns + tps - 1) / tps * tps);
 }
 
-// The seeded chain of one launch shape over a packed base: launch l covers
-// tiles [t0[l], t0[l + 1]) and writes its partials from slab p0[l].
+// The seeded chain over a packed base: launch l covers tiles
+// [t0[l], t0[l + 1]) with shape shp[l] (groups[l] query groups) and writes its
+// partials from slab p0[l]; every launch uses the common query stride.
 struct ChainPlan {
     static constexpr int MAX_LAUNCH = 8;
-    int n_launch = 0, n_groups = 0;
+    int n_launch = 0;
     uint32_t q_stride = 0;
+    const FusedShape* shp[MAX_LAUNCH];
+    int groups[MAX_LAUNCH];
     int t0[MAX_LAUNCH + 1], slabs[MAX_LAUNCH], tps_slab[MAX_LAUNCH], p0[MAX_LAUNCH + 1];
     size_t part_bytes() const { return (size_t)p0[n_launch] * q_stride * sizeof(uint4); }
     size_t bound_bytes() const { return (size_t)q_stride * sizeof(int); }
 };
 
+// The first launch of a chain runs unbounded over few tiles (128), so its
+// blocks are few and each wave spends most of its time in the slow path.
+// There the kind's listed shape with one query block per wave (same waves and
+// stage depth), where there is one, runs twice the blocks at half the work
+// each, with more waves per SIMD: 25 against 33 us on the 1M x 10k BF16
+// chain.  Later launches keep the chain's shape (one query block was no
+// faster there), and so does I8, whose chain shape already holds three
+// blocks per CU (its one-block shape ran that launch in 28 against 26 us).
+// SIFT_SEED_SHAPE="qw,waves,tps" overrides it with another listed shape.
+const FusedShape* first_launch_shape(const FusedShape& sh) {
+    int qw = 1, waves = sh.waves, tps = sh.tps;
+    if (const char* env = std::getenv("SIFT_SEED_SHAPE")) {
+        if (std::sscanf(env, "%d,%d,%d", &qw, &waves, &tps) != 3) return &sh;
+    }
+    const FusedShape* cand[16];
+    const int nc = shapes_of_kind(sh.kind, cand, 16);
+    for (int i = 0; i < nc; i++)
+        if (cand[i]->qw == qw && cand[i]->waves == waves && cand[i]->tps == tps) return cand[i];
+    return &sh;
+}
+
 ChainPlan plan_chain(const FusedShape& sh, uint32_t n_tiles, uint32_t q_n, uint32_t r_n) {
     ChainPlan c;
-    const int qpg = sh.qw * sh.waves * 32;
-    c.n_groups = (int)((q_n + qpg - 1) / qpg);
-    c.q_stride = (uint32_t)c.n_groups * qpg;
-    const int cap = device_cu_count() * sh.blocks_per_cu();
+    const FusedShape* first = first_launch_shape(sh);
     c.n_launch = 1 + (r_n >= 2 ? seed_ends((int)n_tiles, c.t0 + 1, ChainPlan::MAX_LAUNCH - 1) : 0);
     c.t0[0] = 0;
     c.t0[c.n_launch] = (int)n_tiles;
     c.p0[0] = 0;
     for (int l = 0; l < c.n_launch; l++) {
-        choose_slabs(c.t0[l + 1] - c.t0[l], c.n_groups, cap, sh.tps, &c.slabs[l], &c.tps_slab[l]);
+        const int nt = c.t0[l + 1] - c.t0[l];
+        const FusedShape* s = (c.n_launch > 1 && l == 0) ? first : &sh;
+        const int qpg = s->qw * s->waves * 32;
+        c.shp[l] = s;
+        c.groups[l] = (int)((q_n + qpg - 1) / qpg);
+        c.q_stride = std::max(c.q_stride, (uint32_t)c.groups[l] * qpg);
+        const int cap = device_cu_count() * s->blocks_per_cu();
+        choose_slabs(nt, c.groups[l], cap, s->tps, &c.slabs[l], &c.tps_slab[l]);
         c.p0[l + 1] = c.p0[l] + c.slabs[l];
     }
     return c;
@@ -371,7 +399,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
     const ChainPlan c = plan_chain(sh, bs.ntiles, q_n, r_n);
     if (c.q_stride > qs.rows_pad)
         return set_error(h, SIFT_ERR_INVALID, "packed query buffer smaller than the launch shape");
-    const int n_groups = c.n_groups, n_launch = c.n_launch;
+    const int n_launch = c.n_launch;
     const uint32_t q_stride = c.q_stride;
     const int* t0 = c.t0;
     const int* slabs = c.slabs;
@@ -411,7 +439,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
         fa.end_tile = t0[l + 1];
         fa.tiles_per_slab = tps_slab[l];
         fa.n_slabs = slabs[l];
-        fa.n_groups = n_groups;
+        fa.n_groups = c.groups[l];
         fa.wait_polls = h->wait_polls;
         fa.status = wait_status(h);
         fa.coarse = coarse;
@@ -456,7 +484,7 @@ int launch_fused_shape(sift_context* h, const FusedShape& sh, const PackedSet& q
             SIFT_HIP_TRY(h, hipGetLastError());
         }
         ProfScope ps(h, PROF_FUSED);
-        sh.fn(mk_fused(l), (unsigned)(slabs[l] * n_groups), h->stream);
+        c.shp[l]->fn(mk_fused(l), (unsigned)(slabs[l] * c.groups[l]), h->stream);
         SIFT_HIP_TRY(h, hipGetLastError());
     }
     if (bs.kind == KIND_FP8) {  // 16 coarse candidates per query, exact re-rank
