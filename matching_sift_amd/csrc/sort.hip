@@ -6,7 +6,9 @@
 // every table is sorted on the device, each in its own segment of the pair
 // arrays, in ceil(bits/8) passes of:
 //   hist    : per-block 256-bin digit histogram (LDS atomics), bin-major
-//   scan    : exclusive scan of the histogram -> global digit offsets
+//   scan    : exclusive scan of the histogram -> global digit offsets (up to
+//             2^20 counts: one launch scans 4096-count blocks and the scatter
+//             adds the carries between blocks)
 //   scatter : stable scatter; ranks among equal digits come from wave-wide
 //             ballots, so elements keep their input order within a digit;
 //             each chunk is ordered by digit in LDS and written run by run.
@@ -44,38 +46,81 @@ __device__ inline SegChunk seg_chunk(uint32_t blk, uint32_t seg_n, uint32_t nbt)
     return c;
 }
 
+// Per-wave 256-bin counts in LDS (less contention than one shared set), keys
+// read 16 bytes per lane where the chunk is whole and 16-byte aligned.
 __global__ __launch_bounds__(RS_T) void rs_hist(const uint32_t* __restrict__ keys, uint32_t seg_n, uint32_t nbt,
                                                 int shift, uint32_t dmask, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t cnt[256];
-    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+    __shared__ uint32_t cnt[RS_W][256];
+    for (int i = threadIdx.x; i < RS_W * 256; i += RS_T) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
+    uint32_t* wc = cnt[threadIdx.x >> 6];
+    if (c.m == (uint32_t)RS_CHUNK && (c.base & 3) == 0) {
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + c.base);
+        uint4 v[RS_I / 4];
 #pragma unroll
-    for (int i = 0; i < RS_I; i++) {
-        const uint32_t idx = (uint32_t)i * RS_T + threadIdx.x;
-        if (idx < c.m) atomicAdd(&cnt[(keys[c.base + idx] >> shift) & dmask], 1u);
+        for (int i = 0; i < RS_I / 4; i++) v[i] = k4[i * RS_T + threadIdx.x];
+#pragma unroll
+        for (int i = 0; i < RS_I / 4; i++) {
+            atomicAdd(&wc[(v[i].x >> shift) & dmask], 1u);
+            atomicAdd(&wc[(v[i].y >> shift) & dmask], 1u);
+            atomicAdd(&wc[(v[i].z >> shift) & dmask], 1u);
+            atomicAdd(&wc[(v[i].w >> shift) & dmask], 1u);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < RS_I; i++) {
+            const uint32_t idx = (uint32_t)i * RS_T + threadIdx.x;
+            if (idx < c.m) atomicAdd(&wc[(keys[c.base + idx] >> shift) & dmask], 1u);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < 256) hist[c.hrow + (size_t)threadIdx.x * nbt] = cnt[threadIdx.x];
+    if (threadIdx.x < 256) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int k = 0; k < RS_W; k++) t += cnt[k][threadIdx.x];
+        hist[c.hrow + (size_t)threadIdx.x * nbt] = t;
+    }
 }
 
 // Stable scatter of one chunk: ranks come from wave-wide ballots, the chunk
 // is first ordered by digit in LDS, then written out with consecutive
 // threads on consecutive positions of each digit's run (whole runs instead
 // of scattered 4-byte writes).  vin == nullptr: the value of each element is
-// its position in its segment.
+// its position in its segment.  offs: the digit counts scanned within
+// 4096-entry blocks; sums (n_sums <= 256 block totals, in block order) carry
+// them across blocks here, or null when offs is fully scanned.
+constexpr int RS_SUMS_MAX = 256;
 __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ kin,
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout, uint32_t seg_n, uint32_t nbt,
-                                                   int shift, int dbits, const uint32_t* __restrict__ offs) {
+                                                   int shift, int dbits, const uint32_t* __restrict__ offs,
+                                                   const uint32_t* __restrict__ sums, uint32_t n_sums) {
     const uint32_t dmask = (1u << dbits) - 1u;  // the pass's significant digit bits
     __shared__ uint32_t sk[RS_CHUNK], sv[RS_CHUNK];
     __shared__ uint32_t wh[RS_W][256];  // per-wave digit counts, then local cursors
     __shared__ uint32_t delta[256], wsum[4];  // per digit: output position - position in the chunk
+    __shared__ uint32_t carry[RS_SUMS_MAX], csum[4];  // exclusive prefix of the block totals
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < RS_W * 256; i += RS_T) (&wh[0][0])[i] = 0;
+    uint32_t cx = 0, ct = 0;
+    if (sums && threadIdx.x < RS_SUMS_MAX) {  // waves 0..3: inclusive scan of the totals
+        ct = threadIdx.x < n_sums ? sums[threadIdx.x] : 0u;
+        cx = ct;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(cx, o);
+            if (lane >= o) cx += y;
+        }
+        if (lane == 63) csum[w] = cx;
+    }
     __syncthreads();
+    if (sums && threadIdx.x < RS_SUMS_MAX) {
+        uint32_t pre = 0;
+        for (int k = 0; k < w; k++) pre += csum[k];
+        carry[threadIdx.x] = pre + cx - ct;
+    }
     const SegChunk c = seg_chunk(blockIdx.x, seg_n, nbt);
     const uint32_t sub = (uint32_t)w * RS_SUB;  // the wave's part of the chunk
     uint32_t key[RS_STEPS];  // read once, kept for the ranking pass
@@ -108,7 +153,9 @@ __global__ __launch_bounds__(RS_T) void rs_scatter(const uint32_t* __restrict__ 
         uint32_t pre = 0;
         for (int k = 0; k < w; k++) pre += wsum[k];
         const uint32_t ls = pre + x - tot;
-        delta[bin] = offs[c.hrow + (size_t)bin * nbt] - ls;  // modulo 2^32
+        const size_t g = c.hrow + (size_t)bin * nbt;
+        const uint32_t o = offs[g] + (sums ? carry[g >> 12] : 0u);
+        delta[bin] = o - ls;  // modulo 2^32
         uint32_t run = ls;
 #pragma unroll
         for (int k = 0; k < RS_W; k++) {
@@ -347,14 +394,27 @@ int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t
     if (nblk * 256 >= (1ull << 32)) return set_error(h, SIFT_ERR_UNSUPPORTED, "sort: too many chunks");
     SIFT_TRY(reserve(h, hist, (size_t)nblk * 256 * 4));
     auto* hp = static_cast<uint32_t*>(hist.p);
+    const uint32_t n_cnt = (uint32_t)(nblk * 256);
+    const uint32_t n_sums = (n_cnt + 4095) / 4096;
+    // up to 256 scan blocks (2^20 counts, 32M pairs): one scan launch, and the
+    // scatter carries the block totals itself
+    const bool carry = n_sums <= (uint32_t)RS_SUMS_MAX;
+    if (carry) SIFT_TRY(reserve(h, levels[0], (size_t)n_sums * 4));
+    auto* sums = carry ? static_cast<uint32_t*>(levels[0].p) : nullptr;
     for (int shift = 0; shift < std::max(bits, 1); shift += 8) {
         // digit bits at or above `bits` are constant within a segment: masked off
         const int dbits = std::min(8, std::max(bits, 1) - shift);
         rs_hist<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, seg_n, nbt, shift, (1u << dbits) - 1u, hp);
         SIFT_HIP_TRY(h, hipGetLastError());
-        SIFT_TRY(exclusive_scan(h, hp, (uint32_t)(nblk * 256), levels, 0));
+        if (carry) {
+            scan_blocks<<<n_sums, 256, 0, h->stream>>>(hp, n_cnt, sums);
+            SIFT_HIP_TRY(h, hipGetLastError());
+        } else {
+            SIFT_TRY(exclusive_scan(h, hp, n_cnt, levels, 0));
+        }
         const uint32_t* vin = implicit_vals && shift == 0 ? nullptr : *vals;
-        rs_scatter<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, vin, *tk, *tv, seg_n, nbt, shift, dbits, hp);
+        rs_scatter<<<(unsigned)nblk, RS_T, 0, h->stream>>>(*keys, vin, *tk, *tv, seg_n, nbt, shift, dbits, hp,
+                                                            sums, n_sums);
         SIFT_HIP_TRY(h, hipGetLastError());
         std::swap(*keys, *tk);
         std::swap(*vals, *tv);
