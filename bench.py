@@ -520,6 +520,10 @@ def main():
     # together the launches of one step cover the base once
     fused_step_s = fused_ms / args.steps / 1e3
     flops = 2.0 * M * N * 128
+    import ctypes as C
+    sq, sw, st = C.c_int(), C.c_int(), C.c_int()  # the launch shape the timed steps ran
+    ms._lib.check(L.sift_knn2_shape(h.h, compute, N, C.byref(sq), C.byref(sw), C.byref(st)), h.h)
+    shape = [sq.value, sw.value, st.value]
     achieved = flops / fused_step_s / 1e12
     peak = PEAK[args.compute] / 1e12
 
@@ -608,7 +612,7 @@ def main():
         "data": "synthetic (seeded SIFT-like integers in [0,255], float32 in HBM)",
         "config": {"workload": "exact 2-NN brute force, base N=%d, queries M=%d, d=128, "
                                "float32 inputs, %s MFMA, fused distance+top-2" % (N, M, args.compute),
-                   "n_base": N, "n_query": M, "dim": 128, "compute": args.compute},
+                   "n_base": N, "n_query": M, "dim": 128, "compute": args.compute, "shape": shape},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                      "unit": "TOP/s" if args.compute == "i8" else "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,

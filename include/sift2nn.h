@@ -120,6 +120,12 @@ int sift_memset(void* dst_dev, int value, size_t bytes);
 int sift_profile_enable(sift_handle_t handle, int on);
 int sift_profile_read(sift_handle_t handle, const char* kernel, double* total_ms, long* launches);
 int sift_profile_reset(sift_handle_t handle);
+/* The brute-force launch shape (query blocks per wave, waves per block, tiles
+ * per stage) that calls of this compute kind (not AUTO) use on a base of r_n
+ * rows: the one the handle's first such call timed, or the kind's default
+ * before that.  No counterpart in the reference: for benchmark and tuning
+ * reports. */
+int sift_knn2_shape(sift_handle_t handle, int compute, uint32_t r_n, int* qw, int* waves, int* tps);
 
 /* ------------------------------------------------ reference entry points -- */
 

@@ -44,6 +44,7 @@ SIGNATURES = {
     "sift_memset": (i32, [vp, i32, sz]),
     "sift_profile_enable": (i32, [vp, i32]),
     "sift_profile_read": (i32, [vp, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_long)]),
+    "sift_knn2_shape": (i32, [vp, i32, u32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     "sift_profile_reset": (i32, [vp]),
     "cublas_2nn_sift": (i32, [vp, vp, i32, u32, u32, vp, f32, vp, i32, sz, i32]),
     "cublas_2nn_sift_batch": (i32, [vp, vp, i32, vp, u32, u32, vp, vp, f32, vp, vp, i32]),

@@ -717,8 +717,8 @@ const FusedShape kShapes[] = {
     SHAPE(KIND_FP8, 4, 4, 2),
     SHAPE(KIND_FP8, 2, 4, 2),
     SHAPE(KIND_FP8, 2, 8, 2),
-    SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 4, 4, 2),
+    SHAPE(KIND_I8, 2, 4, 2),
     SHAPE(KIND_I8, 2, 8, 2),
 };
 

@@ -275,8 +275,10 @@ int run_sequence(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx
     return SIFT_OK;
 }
 
-// Time every listed shape of the kind on the real data (two launches each,
-// the second timed) and keep the fastest.
+// Time every listed shape of the kind on the real data and keep the one with
+// the lowest total: the shapes in listed order (an untimed launch, then a
+// timed one each), then in reverse order (one timed launch each), so that a
+// clock still rising on the first execute favours no shape by its position.
 int autotune(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx, float* dist,
              uint32_t* matches, float ratio) {
     sift_context* h = p->h;
@@ -290,33 +292,36 @@ int autotune(sift_pairs_plan* p, const void* q, const void* r, uint32_t* idx, fl
         (void)hipEventDestroy(e0);
         return set_error(h, SIFT_ERR_HIP, "hipEventCreate");
     }
-    const FusedShape* best = p->shape;
-    float best_ms = FLT_MAX;
+    float sum_ms[16];
+    for (int i = 0; i < nc; i++) sum_ms[i] = 0.f;
     int rc = SIFT_OK;
-    for (int i = 0; i < nc && rc == SIFT_OK; i++) {
-        p->shape = cand[i];
-        rc = build_tasks(p);
-        if (rc != SIFT_OK || p->n_tasks == 0) break;
-        const FusedArgs a = make_args(p, idx, dist, matches, ratio);
-        p->shape->pairs_fn(a, p->grid, h->stream);
-        (void)hipEventRecord(e0, h->stream);
-        p->shape->pairs_fn(a, p->grid, h->stream);
-        (void)hipEventRecord(e1, h->stream);
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-            rc = set_error(h, SIFT_ERR_HIP, "pairs autotune launch failed");
-            break;
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = cand[i];
+    for (int pass = 0; pass < 2 && rc == SIFT_OK; pass++) {
+        for (int k = 0; k < nc && rc == SIFT_OK; k++) {
+            const int i = pass ? nc - 1 - k : k;
+            p->shape = cand[i];
+            rc = build_tasks(p);
+            if (rc != SIFT_OK || p->n_tasks == 0) break;
+            const FusedArgs a = make_args(p, idx, dist, matches, ratio);
+            if (pass == 0) p->shape->pairs_fn(a, p->grid, h->stream);
+            (void)hipEventRecord(e0, h->stream);
+            p->shape->pairs_fn(a, p->grid, h->stream);
+            (void)hipEventRecord(e1, h->stream);
+            if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = set_error(h, SIFT_ERR_HIP, "pairs autotune launch failed");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            sum_ms[i] += ms;
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     SIFT_TRY(rc);
-    p->shape = best;
+    int best = 0;
+    for (int i = 1; i < nc; i++)
+        if (sum_ms[i] < sum_ms[best]) best = i;
+    p->shape = cand[best];
     return build_tasks(p);
 }
 
