@@ -34,7 +34,6 @@
 
 namespace sift {
 int exclusive_scan(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
-int suffix_min(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth);
 int scan_reserve(sift_context* h, uint32_t n, DevBuf* levels);
 int sort_reserve(sift_context* h, uint32_t seg_n, uint32_t nseg, DevBuf& hist, DevBuf* levels);
 int radix_sort_pairs(sift_context* h, uint32_t** keys, uint32_t** vals, uint32_t** tk, uint32_t** tv,
@@ -814,7 +813,8 @@ int check_flag(sift_context* h, int* flag, const char* what) {
 
 // Bucket starts for direct lookup: offs[t][c] = first position of table t's
 // sorted range [t*n, (t+1)*n) whose key is >= (t << b | c); offs[t][2^b] is
-// the table's end.  One thread per (table, code), each a binary search.
+// the table's end.  Small table sets: one thread per (table, code), each a
+// binary search; larger ones: one pass over the sorted keys (offs_fill).
 constexpr int OFFS_MAX_BITS = 20;
 constexpr uint32_t OFFS_SEARCH_MAX = 1u << 20;  // entries up to which bucket starts are searched
 // Bucket starts [L][2^b + 1] of a small table set: one thread per entry, a
@@ -838,24 +838,53 @@ __global__ void bucket_offsets(const uint32_t* __restrict__ keys, uint32_t n, in
     offs[i] = lo;
 }
 
-// The same for large table sets, from the sorted keys: offs_init sets every
-// entry past the end (all ones) and each table's last entry to its end
-// (t + 1) n; offs_mark writes the first position of every present code;
-// a suffix minimum then gives each absent code the start of the next present
-// one (an empty bucket).  Tables lie in order with increasing positions, so
-// one suffix minimum over the whole array never crosses into another table.
-__global__ void offs_init(uint32_t* __restrict__ offs, uint32_t per, uint32_t total, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < total) offs[i] = (i % per == per - 1) ? (i / per + 1) * n : 0xFFFFFFFFu;
-}
-
-__global__ void offs_mark(const uint32_t* __restrict__ keys, uint32_t n, uint32_t mask, uint32_t per,
-                          uint32_t* __restrict__ offs) {
-    const uint32_t t = blockIdx.y, i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// The same for large table sets (b = 20: 2^20 entries and more per index), in
+// one pass over the sorted keys: the thread of position i of table t (key
+// code k, previous code p) writes t*n + i to the entries of every code in
+// (p, k] (in [0, k] at i = 0), and the thread of position n writes the
+// table's end to those after the last code, entry 2^b included, so every
+// entry is written exactly once.  Gaps of up to 4 codes are written by their
+// thread (a wave loops as long as its longest such gap); longer ones are
+// queued in LDS and each written by one wave of the block, 64 entries per
+// store.  (One thread per entry with a binary search, as below 2^20
+// entries, costs ~20 dependent loads per entry here; a load-balanced fill
+// of the block's entries stalls on blocks whose positions own long runs of
+// empty codes, which clustered codes produce.)
+constexpr uint32_t FILL_OWN = 4;
+__global__ __launch_bounds__(256) void offs_fill(const uint32_t* __restrict__ keys, uint32_t n, uint32_t mask,
+                                                 uint32_t per, uint32_t* __restrict__ offs) {
+    __shared__ uint32_t q_lo[256], q_hi[256], q_v[256];
+    __shared__ uint32_t q_n;
+    if (threadIdx.x == 0) q_n = 0;
+    __syncthreads();
+    const uint32_t t = blockIdx.y, i = blockIdx.x * 256u + threadIdx.x;  // positions 0..n
     const uint32_t* tk = keys + (size_t)t * n;
-    const uint32_t k = tk[i];
-    if (i == 0 || tk[i - 1] != k) offs[(size_t)t * per + (k & mask)] = t * n + i;
+    uint32_t* to = offs + (size_t)t * per;
+    const uint32_t v = t * n + i;
+    uint32_t lo = 0, hi = 0;  // codes [lo, hi) take v
+    if (i <= n) {
+        const uint32_t prev = i > 0 ? (tk[i - 1] & mask) : 0u;
+        const uint32_t cur = i < n ? (tk[i] & mask) : 0u;
+        if (i == n) {
+            lo = prev + 1u;  // n > 0 here
+            hi = per;
+        } else if (i == 0 || cur != prev) {
+            lo = i == 0 ? 0u : prev + 1u;
+            hi = cur + 1u;
+        }
+    }
+    if (hi - lo <= FILL_OWN) {
+        for (uint32_t c = lo; c < hi; c++) to[c] = v;
+    } else {
+        const uint32_t slot = atomicAdd(&q_n, 1u);
+        q_lo[slot] = lo;
+        q_hi[slot] = hi;
+        q_v[slot] = v;
+    }
+    __syncthreads();
+    const uint32_t nq = q_n, lane = threadIdx.x & 63u;
+    for (uint32_t g = threadIdx.x >> 6; g < nq; g += 4u)
+        for (uint32_t c = q_lo[g] + lane; c < q_hi[g]; c += 64u) to[c] = q_v[g];
 }
 
 // Sum over buckets of size^2 (= sum over sorted positions of the size of the
@@ -992,28 +1021,21 @@ int build_offsets(sift_context* h, sift_lsh_index* ix) {
     const size_t nparts = (size_t)grid.x * grid.y;
     // every buffer first: an allocation between launches would stall the queue
     SIFT_TRY(reserve(h, ix->hist, nparts * 8));
-    if (direct) {
-        SIFT_TRY(reserve(h, ix->offs, (size_t)no * 4));
-        SIFT_TRY(scan_reserve(h, no, ix->levels));
-    }
+    if (direct) SIFT_TRY(reserve(h, ix->offs, (size_t)no * 4));
     auto* part = static_cast<unsigned long long*>(ix->hist.p);
     {
         ProfScope ps(h, PROF_LSH_SORT);  // bucket structures count with the sort
         if (direct) {
             auto* offs = static_cast<uint32_t*>(ix->offs.p);
             const auto* keys = static_cast<const uint32_t*>(ix->keys.p);
-            if (no <= OFFS_SEARCH_MAX) {  // few entries: a search each beats three passes over the keys
+            if (no <= OFFS_SEARCH_MAX) {  // few entries: a search each beats a pass over the keys
                 bucket_offsets<<<(no + 255) / 256, 256, 0, h->stream>>>(keys, ix->n, ix->nbits, (uint32_t)ix->L,
                                                                           offs);
-                SIFT_HIP_TRY(h, hipGetLastError());
             } else {
-                offs_init<<<(no + 255) / 256, 256, 0, h->stream>>>(offs, per, no, ix->n);
-                SIFT_HIP_TRY(h, hipGetLastError());
-                offs_mark<<<dim3((ix->n + 255) / 256, (unsigned)ix->L), 256, 0, h->stream>>>(
+                offs_fill<<<dim3(ix->n / 256 + 1, (unsigned)ix->L), 256, 0, h->stream>>>(
                     keys, ix->n, (1u << ix->nbits) - 1u, per, offs);
-                SIFT_HIP_TRY(h, hipGetLastError());
-                SIFT_TRY(suffix_min(h, offs, no, ix->levels, 0));
             }
+            SIFT_HIP_TRY(h, hipGetLastError());
             bucket_sq_offs<<<grid, 256, 0, h->stream>>>(offs, ix->nbits, (uint32_t)ix->L, part);
         } else {
             bucket_sq<<<grid, 256, 0, h->stream>>>(static_cast<const uint32_t*>(ix->keys.p), ix->n, part);
@@ -1111,7 +1133,6 @@ int build_tables(sift_context* h, sift_lsh_index* ix, const void* r, int type, i
         if (r_n && ix->nbits <= OFFS_MAX_BITS) {
             const uint32_t no = (uint32_t)ix->L * ((1u << ix->nbits) + 1u);
             if ((rc = reserve(h, ix->offs, (size_t)no * 4)) != SIFT_OK) break;
-            if ((rc = scan_reserve(h, no, ix->levels)) != SIFT_OK) break;
         }
         auto* keys = static_cast<uint32_t*>(ix->keys.p);
         if (!force_float && ix->hash_kind != SIFT_LSH_LEARNED) {

@@ -267,63 +267,6 @@ __global__ __launch_bounds__(256) void scan_add(uint32_t* __restrict__ data, uin
         if (base + i < n) data[base + i] += add;
 }
 
-// In-place suffix minimum of 4096-element blocks (each entry becomes the
-// minimum of itself and every later entry of its block); block minima to
-// mins[].
-__global__ __launch_bounds__(256) void suffix_min_blocks(uint32_t* __restrict__ data, uint32_t n,
-                                                         uint32_t* __restrict__ mins) {
-    __shared__ uint32_t wmin[4];
-    const size_t base = (size_t)blockIdx.x * 4096 + (size_t)threadIdx.x * 16;
-    uint32_t v[16];
-    if (base + 16 <= n) {  // whole: four 16-byte loads
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint4 u = reinterpret_cast<const uint4*>(data + base)[i];
-            v[4 * i] = u.x; v[4 * i + 1] = u.y; v[4 * i + 2] = u.z; v[4 * i + 3] = u.w;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = (base + i < n) ? data[base + i] : 0xFFFFFFFFu;
-    }
-#pragma unroll
-    for (int i = 14; i >= 0; i--) v[i] = min(v[i], v[i + 1]);
-    // inclusive suffix minimum of the thread minima across the wave (lanes >= this one)
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t x = v[0];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_down(x, o);
-        if (lane + o < 64) x = min(x, y);
-    }
-    if (lane == 0) wmin[w] = x;
-    __syncthreads();
-    uint32_t after = __shfl_down(x, 1);  // lanes after this one
-    if (lane == 63) after = 0xFFFFFFFFu;
-    for (int k = w + 1; k < 4; k++) after = min(after, wmin[k]);
-#pragma unroll
-    for (int i = 0; i < 16; i++) v[i] = min(v[i], after);
-    if (base + 16 <= n) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            reinterpret_cast<uint4*>(data + base)[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            if (base + i < n) data[base + i] = v[i];
-    }
-    if (threadIdx.x == 0) mins[blockIdx.x] = min(min(wmin[0], wmin[1]), min(wmin[2], wmin[3]));
-}
-
-// Entries of block b take the minimum of every later block (suffix[b + 1]).
-__global__ __launch_bounds__(256) void suffix_min_apply(uint32_t* __restrict__ data, uint32_t n,
-                                                        const uint32_t* __restrict__ suffix, uint32_t nb) {
-    if (blockIdx.x + 1 >= nb) return;
-    const size_t base = (size_t)blockIdx.x * 4096;
-    const uint32_t m = suffix[blockIdx.x + 1];
-    for (int i = threadIdx.x; i < 4096; i += 256)
-        if (base + i < n) data[base + i] = min(data[base + i], m);
-}
-
 }  // namespace
 
 // Scratch of a scan of n entries: one level of block totals per 4096x
@@ -333,23 +276,6 @@ int scan_reserve(sift_context* h, uint32_t n, DevBuf* levels) {
         const uint32_t nb = (n + 4095) / 4096;
         SIFT_TRY(reserve(h, levels[depth], (size_t)nb * 4));
         n = nb;
-    }
-    return SIFT_OK;
-}
-
-// In-place suffix minimum of n uint32 (recursive over 4096-element blocks).
-int suffix_min(sift_context* h, uint32_t* data, uint32_t n, DevBuf* levels, int depth) {
-    if (n == 0) return SIFT_OK;
-    if (depth >= 4) return set_error(h, SIFT_ERR_UNSUPPORTED, "scan too deep");
-    const uint32_t nb = (n + 4095) / 4096;
-    SIFT_TRY(reserve(h, levels[depth], (size_t)nb * 4));
-    auto* mins = static_cast<uint32_t*>(levels[depth].p);
-    suffix_min_blocks<<<nb, 256, 0, h->stream>>>(data, n, mins);
-    SIFT_HIP_TRY(h, hipGetLastError());
-    if (nb > 1) {
-        SIFT_TRY(suffix_min(h, mins, nb, levels, depth + 1));
-        suffix_min_apply<<<nb, 256, 0, h->stream>>>(data, n, mins, nb);
-        SIFT_HIP_TRY(h, hipGetLastError());
     }
     return SIFT_OK;
 }
