@@ -159,6 +159,9 @@ struct PairTask {
     int32_t slot;  // FP8 coarse pass: which quarter of the base image (candidate slot)
 };
 
+// Chunks per region of a launch with guided slabs (at most).
+constexpr int MAX_CHUNKS = 32;
+
 struct FusedArgs {
     const uint4* q_tiles;   // packed queries, [qblocks][G][64]
     const uint4* b_tiles;   // packed base, [n_tiles][G][64]
@@ -173,6 +176,14 @@ struct FusedArgs {
     // brute-force kernel: slab s covers tiles [tile0 + s*tiles_per_slab, ...)
     // clipped to end_tile (absolute tile numbers, so row indices are global)
     int tile0, end_tile, tiles_per_slab, n_slabs, n_groups;
+    // Guided slabs (n_chunks > 0, instead of tiles_per_slab): the range
+    // [tile0, end_tile) is cut into 8 regions of `region` tiles, one per XCD,
+    // and region x into n_chunks chunks, tiles [chunk_start[c],
+    // chunk_start[c + 1]) of it, that shrink towards its end.  Block b runs
+    // on XCD x = b % 8 and covers chunk c = (b / 8) / n_groups of region x for
+    // group (b / 8) % n_groups, writing slab x * n_chunks + c.
+    int region, n_chunks;
+    int chunk_start[MAX_CHUNKS + 1];
     // pairs kernel only
     const PairTask* tasks;
     int n_tasks;

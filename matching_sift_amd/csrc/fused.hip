@@ -625,12 +625,25 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     // groups of one slab are placed on the same XCD to share its L2 copy
     const int b = blockIdx.x;
     const int xcd = b & 7, r = b >> 3;
-    const int slab = (r / a.n_groups) * 8 + xcd;
+    const int k = r / a.n_groups;  // the XCD's k-th slab
     const int group = r % a.n_groups;
-    if (slab >= a.n_slabs) return;  // uniform per block
     SlabTask t;
-    t.t_begin = a.tile0 + slab * a.tiles_per_slab;
-    t.t_end = min(t.t_begin + a.tiles_per_slab, a.end_tile);
+    int slab;
+    if (a.n_chunks > 0) {
+        // guided: XCD x covers region x in chunks that shrink with k; the
+        // dispatcher starts blocks in index order, so each XCD ends on short
+        // blocks instead of a last round of full slabs
+        if (k >= a.n_chunks) return;  // uniform per block
+        const int rbase = a.tile0 + xcd * a.region;
+        t.t_begin = min(rbase + a.chunk_start[k], a.end_tile);
+        t.t_end = min(rbase + a.chunk_start[k + 1], a.end_tile);
+        slab = xcd * a.n_chunks + k;
+    } else {
+        slab = k * 8 + xcd;
+        if (slab >= a.n_slabs) return;  // uniform per block
+        t.t_begin = a.tile0 + slab * a.tiles_per_slab;
+        t.t_end = min(t.t_begin + a.tiles_per_slab, a.end_tile);
+    }
     t.qblk0 = group * WAVES * QW;
     t.r_lim = a.r_n;
     t.row_off = 0;
