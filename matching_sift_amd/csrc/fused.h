@@ -186,13 +186,14 @@ struct FusedArgs {
     int chunk_start[MAX_CHUNKS + 1];
     // pairs kernel only
     const PairTask* tasks;
+    const int32_t* task_off;  // block b runs tasks [task_off[b], task_off[b + 1])
     int n_tasks;
     const float* q_norm;  // packed query norms (int32 bits for KIND_I8)
     uint32_t* out_idx;
     float* out_dist;
     uint32_t* out_matches;
     float ratio;
-    int comp_mul;  // pairs kernel, I8: 128, passed at run time so a composite takes one v_mad_i32_i24
+    int comp_mul;  // pairs kernel, I8: 256, passed at run time so a composite takes one v_mad_i32_i24
     uint32_t* out_cand;  // FP8 coarse pass: 16 candidate rows per query
     // 8-wave blocks: polls allowed per stage hand-off wait, and the status word
     // a wave ORs 1 into when a wait runs out of them (its results are then void)

@@ -224,10 +224,16 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     key_t thr_h[QW];
     uint32_t i1[QW], i2[QW];
     // Packed mode (pairs kernel, integer kinds): each lane keeps its top-2 as
-    // int32 "composites" key*64 + slot so one v_min + one v_med3 per row
-    // maintain it with no branch (see the insert below).
+    // int32 "composites" key*F + slot so one v_min + one v_med3 per row
+    // maintain it with no branch (see the insert below).  With two tiles per
+    // stage the slot also carries the tile (RB = 1), and new entries are
+    // recorded once per stage instead of once per tile.
     constexpr bool COMP = PAIRS && KIND != KIND_F32;
-    constexpr int CNONE = 0x7FFFFFC0;  // above every real composite, low bits 0
+    constexpr int RB = TPS == 2 ? 1 : 0;     // tile bits in a slot
+    constexpr int SB = 6 + RB;               // slot bits: F = 2^SB
+    constexpr int NEWB = 1 << (SB - 1);      // set in the slot of an entry not yet recorded
+    constexpr int CNONE = 0x7FFFFFFF & ~((1 << SB) - 1);  // above every real composite, slot bits 0
+    static_assert(!(PAIRS && KIND == KIND_I8) || TPS == 2, "I8 pair composites carry one tile bit");
     int c1[QW], c2[QW];
     const uint32_t lq = (uint32_t)(lane & 31);
 #pragma unroll
@@ -381,14 +387,18 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 
         if constexpr (COMP) {
             // Every row is inserted, branch-free.  A row's composite is
-            // key*64 + 32 + r (r = the row within its 32-row tile); entries
-            // kept from earlier tiles are stored as key*64 (slot bits cleared).
-            // Order by composite = order by (key, row): a new row loses a tie
-            // against a kept one (earlier tile, lower index) and wins or loses
-            // within its tile by r.  Keys are exact integers below 2^24 in
-            // magnitude for SIFT data, so key*64 + 63 fits in int32.  I8: the
-            // packed base norms arrive as (c << 6) | (32 + r) (compose_slots),
-            // so a composite is one v_mad_i32_i24: 128 q''.b' + that.
+            // key*F + NEWB + 32*tt + r (r = the row within its 32-row tile, tt
+            // = the tile within the stage when RB = 1); entries kept from
+            // earlier stages are stored as key*F (slot bits cleared).  Order
+            // by composite = order by (key, row): a new row loses a tie
+            // against a kept one (earlier stage, lower index) and wins or
+            // loses within its stage by 32*tt + r.  Keys are exact integers
+            // with |key| < 2^24 for SIFT data (base pad rows: 65504 * 256), so
+            // key*128 + 127 fits in int32.  I8: the packed base norms arrive as
+            // (c << SB) | (NEWB + 32 * parity + r) (compose_slots; base images
+            // start on even tiles and stages on even tiles of their image, so
+            // the parity is tt), and a composite is one v_mad_i32_i24:
+            // 2F q''.b' + that.
 #pragma unroll
             for (int tt = 0; tt < TPS; tt++) {
                 if (tt >= ntile) break;  // uniform: slab end
@@ -396,14 +406,13 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                 tile_acc(tt, acc);
                 // base pad rows carry PAD_KEY, so they need no mask here: they
                 // lose to every real row and are dropped after the loop
-                const uint32_t tile_row0 = (uint32_t)(ts + tt) * 32u;
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) {
 #pragma unroll
                     for (int j = 0; j < 16; j++) {
                         int cv;
                         if constexpr (FLOAT_KEY)
-                            cv = ((int)acc[qb][j] << 6) | (32 + 8 * (j >> 2) + 4 * h + (j & 3));
+                            cv = ((int)acc[qb][j] << SB) | (NEWB + 32 * (RB ? tt : 0) + 8 * (j >> 2) + 4 * h + (j & 3));
                         else
                             cv = __mul24((int)acc[qb][j], a.comp_mul) + nlr[tt][j];
                         // c1 <= c2 always, so the new 2nd smallest of the three
@@ -412,15 +421,22 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                         c2[qb] = max(lo, min(hi, cv));  // v_med3_i32
                         c1[qb] = min(c1[qb], cv);
                     }
-                    // new entries carry slot bits: record their rows, then clear
-                    const bool n1 = (c1[qb] & 32) != 0, n2 = (c2[qb] & 32) != 0;
-                    const uint32_t r1 = tile_row0 + (uint32_t)(c1[qb] & 31);
-                    const uint32_t r2 = tile_row0 + (uint32_t)(c2[qb] & 31);
-                    // a kept 2nd entry is the old 1st if the 1st was displaced
-                    i2[qb] = n2 ? r2 : (n1 ? i1[qb] : i2[qb]);
-                    i1[qb] = n1 ? r1 : i1[qb];
-                    c1[qb] &= ~63;
-                    c2[qb] &= ~63;
+                }
+                // after the stage's last tile (or the slab's): new entries
+                // carry slot bits; record their rows, then clear the bits
+                if (RB == 0 || tt == TPS - 1 || tt + 1 >= ntile) {
+                    const uint32_t row0 = (uint32_t)(ts + (RB ? 0 : tt)) * 32u;
+#pragma unroll
+                    for (int qb = 0; qb < QW; qb++) {
+                        const bool n1 = (c1[qb] & NEWB) != 0, n2 = (c2[qb] & NEWB) != 0;
+                        const uint32_t r1 = row0 + (uint32_t)(c1[qb] & (NEWB - 1));
+                        const uint32_t r2 = row0 + (uint32_t)(c2[qb] & (NEWB - 1));
+                        // a kept 2nd entry is the old 1st if the 1st was displaced
+                        i2[qb] = n2 ? r2 : (n1 ? i1[qb] : i2[qb]);
+                        i1[qb] = n1 ? r1 : i1[qb];
+                        c1[qb] &= ~((1 << SB) - 1);
+                        c2[qb] &= ~((1 << SB) - 1);
+                    }
                 }
             }
         } else {
@@ -549,8 +565,8 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         for (int qb = 0; qb < QW; qb++) {
             if (i1[qb] >= task.r_lim) i1[qb] = 0xFFFFFFFFu;
             if (i2[qb] >= task.r_lim) i2[qb] = 0xFFFFFFFFu;
-            m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> 6) : Key<KIND>::none();
-            m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> 6) : Key<KIND>::none();
+            m1[qb] = i1[qb] != 0xFFFFFFFFu ? (key_t)(c1[qb] >> SB) : Key<KIND>::none();
+            m2[qb] = i2[qb] != 0xFFFFFFFFu ? (key_t)(c2[qb] >> SB) : Key<KIND>::none();
         }
     }
     if constexpr (PAIRS && KIND == KIND_FP8) {
@@ -654,12 +670,13 @@ __global__ __launch_bounds__(WAVES * 64) void fused_top2(FusedArgs a) {
     fused_slab<KIND, QW, WAVES, TPS, false>(a, t);
 }
 
-// Persistent kernel over a task list (image pairs): block b takes tasks b,
-// b + grid, ... (the list is sorted by decreasing cost); LDS is reused
-// between tasks behind a barrier.
+// Persistent kernel over a task list (image pairs): block b runs tasks
+// [task_off[b], task_off[b + 1]), a run the host balanced across blocks;
+// LDS is reused between tasks behind a barrier.
 template <int KIND, int QW, int WAVES, int TPS>
 __global__ __launch_bounds__(WAVES * 64) void pairs_top2(FusedArgs a) {
-    for (int it = blockIdx.x; it < a.n_tasks; it += gridDim.x) {
+    const int it_end = a.task_off[blockIdx.x + 1];
+    for (int it = a.task_off[blockIdx.x]; it < it_end; it++) {
         const PairTask pt = a.tasks[it];
         SlabTask t;
         t.t_begin = pt.t_begin;

@@ -234,7 +234,10 @@ __global__ __launch_bounds__(256) void pack_kernel(const void* __restrict__ in, 
             const int x = (int)v[i] - 128;
             si += x * x;
             sb += x;
-            const int op = role == PACK_QUERY ? -x - 1 : x;
+            // base pad rows are zero vectors (b' = 0): their key is exactly
+            // the PAD_KEY norm, so a pairs composite key * 128 + slot of a pad
+            // row stays below 2^31 for every query
+            const int op = role == PACK_QUERY ? -x - 1 : (real || !is_base ? x : 0);
             w[i >> 2] |= ((uint32_t)(op & 0xFF)) << (8 * (i & 3));
         }
         const int g = blk >> 2, h = (blk >> 1) & 1, half = blk & 1;
