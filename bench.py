@@ -56,9 +56,10 @@ def host_info():
     return {"cpu_model": model, "host_logical_cores": os.cpu_count()}
 
 
-def cpu_baseline(q, b, dev_idx, batch, threads, seconds):
+def cpu_baseline(q, b, dev_idx, batch, threads, seconds, dev_dist=None):
     """The oracle's host_brute over batches of `batch` queries against the
-    full base until `seconds` have passed (at least one batch)."""
+    full base until `seconds` have passed (at least one batch).  The GPU's
+    indices (and distances, when given) must equal host_brute's bit for bit."""
     import oracle  # the CPU leg only: timed baseline and checker
     done, ok = 0, True
     t0 = time.perf_counter()
@@ -67,6 +68,8 @@ def cpu_baseline(q, b, dev_idx, batch, threads, seconds):
         n = s.shape[0]
         ok &= bool(np.array_equal(s[:, 2].astype(np.uint32), dev_idx[done:done + n, 0]) and
                    np.array_equal(s[:, 3].astype(np.uint32), dev_idx[done:done + n, 1]))
+        if dev_dist is not None:
+            ok &= bool(np.array_equal(s[:, :2], dev_dist[done:done + n]))
         done += n
     dt = time.perf_counter() - t0
     return done / dt, dt, ok, done
@@ -515,6 +518,17 @@ def main():
         check_idx = ex[0]
     else:
         check_idx = idx
+    check_dist = outs[1].numpy() if compute != ms.COMPUTE_FP8 else None
+    # every query at full size, after the timed region: the exact kinds BF16
+    # and I8 are independent kernels (operand formats, keys, launch shapes)
+    # and must return the same indices and distances bit for bit
+    full_check = None
+    if compute != ms.COMPUTE_FP8:
+        other = ms.COMPUTE_I8 if compute != ms.COMPUTE_I8 else ms.COMPUTE_BF16
+        oi, od, _, _ = ms.knn2(h, qd, bd, compute=other, ratio=0.8)
+        full_check = {"queries": M, "against": "i8" if other == ms.COMPUTE_I8 else "bf16",
+                      "idx_equal": bool(np.array_equal(oi, idx)),
+                      "dist_equal": bool(np.array_equal(od, check_dist))}
     value = M * args.steps / elapsed
     # one call runs fused_top2 over consecutive base ranges (seeded chain);
     # together the launches of one step cover the base once
@@ -583,16 +597,19 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log("cpu baseline: batches of %d queries x %d base on %d threads for ~%.0f s"
             % (args.cpu_queries, N, threads, args.cpu_seconds))
-        qps_cpu, dt, ok, done = cpu_baseline(q, b, check_idx, args.cpu_queries, threads, args.cpu_seconds)
+        qps_cpu, dt, ok, done = cpu_baseline(q, b, check_idx, args.cpu_queries, threads, args.cpu_seconds,
+                                             check_dist)
         cpu = {"value": qps_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
                "sample": "first %d queries x full %d base, host_brute restatement (oracle/), "
-                         "%.1f s; indices equal to the GPU's exact result: %s" % (done, N, dt, ok), **host_info()}
+                         "%.1f s; indices%s equal to the GPU's exact result: %s"
+                         % (done, N, dt, "" if check_dist is None else " and distances", ok), **host_info()}
         # the reference's host_brute as written: one thread
         log("cpu baseline, one thread: batches of 8 queries for ~%.0f s" % (args.cpu_seconds / 3))
-        qps_1, dt_1, ok_1, done_1 = cpu_baseline(q, b, check_idx, 8, 1, args.cpu_seconds / 3)
+        qps_1, dt_1, ok_1, done_1 = cpu_baseline(q, b, check_idx, 8, 1, args.cpu_seconds / 3, check_dist)
         cpu["single_thread"] = {"value": qps_1, "unit": "queries/s", "cores": 1,
-                                "sample": "first %d queries x full %d base, %.1f s; indices equal: %s"
-                                          % (done_1, N, dt_1, ok_1)}
+                                "sample": "first %d queries x full %d base, %.1f s; indices%s equal: %s"
+                                          % (done_1, N, dt_1, "" if check_dist is None else " and distances",
+                                             ok_1)}
 
     out = {
         "metric": ("2-NN queries/sec at 1M x 128 base (exact, bit-exact vs host_brute)"
@@ -628,6 +645,8 @@ def main():
     }
     if recall is not None:
         out["recall_vs_exact"] = recall
+    if full_check is not None:
+        out["full_size_check"] = full_check
     print(json.dumps(out), flush=True)
 
 

@@ -368,14 +368,14 @@ struct ChainPlan {
 // work left per slot (G * R / (K * slots) tiles for R tiles left), between
 // SMIN and SMAX tiles and a multiple of the stage depth.  More than
 // MAX_CHUNKS chunks raise SMIN until the schedule fits.  SIFT_GUIDE="K,SMIN,
-// SMAX" overrides the defaults (1.5, 16, 384); SIFT_GUIDE=0 keeps uniform slabs.
+// SMAX" overrides the defaults (1.5, 16, 768); SIFT_GUIDE=0 keeps uniform slabs.
 bool guided_off() {
     const char* env = std::getenv("SIFT_GUIDE");
     return env && env[0] == '0' && env[1] == 0;
 }
 int guided_schedule(int region, int groups, int slots, int tps, int* starts) {
     double K = 1.5;
-    int smin = 16, smax = 384;
+    int smin = 16, smax = 768;
     if (const char* env = std::getenv("SIFT_GUIDE")) {
         double k = 0;
         int a = 0, b = 0;
