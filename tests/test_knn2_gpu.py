@@ -230,6 +230,33 @@ def test_seed_chain_levels(handle, compute, shape, levels, monkeypatch):
     np.testing.assert_array_equal(mat, mt)
 
 
+@pytest.mark.parametrize("compute", [COMPUTE_BF16, COMPUTE_I8, COMPUTE_F32])
+@pytest.mark.parametrize("guide", ["", "0", "1.5,16,768", "1,16,16", "4,2,8", "0.5,400,4000"])
+def test_guided_slab_schedules(handle, compute, guide, monkeypatch):
+    """The launches after the first split their tile range into one region
+    per XCD and each region into chunks that shrink towards its end.  Every
+    schedule (the default, uniform slabs, many equal small chunks, tiny
+    chunks that hit the chunk cap, one chunk per region) covers every tile
+    once and gives the oracle's result exactly, with row copies at chunk and
+    region edges tying across launches."""
+    if guide:
+        monkeypatch.setenv("SIFT_GUIDE", guide)
+    else:
+        monkeypatch.delenv("SIFT_GUIDE", raising=False)
+    say("guided slabs", compute, repr(guide))
+    rng = np.random.default_rng(21)
+    b = datagen.base(70_000, seed=95)
+    q = datagen.queries(b, 300, seed=96)
+    for lo in (4_000, 8_000, 33_000, 35_040, 52_480, 69_950):
+        b[lo:lo + 30] = b[100:130]
+    q[:30] = b[100:130]
+    q[30:60] = b[rng.integers(0, 70_000, 30)]
+    s, mt = expected(q, b)
+    idx, dist, srt, mat = knn2(handle, q, b, compute=compute, ratio=0.8)
+    np.testing.assert_array_equal(srt, s)
+    np.testing.assert_array_equal(mat, mt)
+
+
 def test_full_size_default_chain(handle):
     """The benchmark's base (1,000,000 rows: a chain of four launches over
     128, 896, 7,168 and 23,058 tiles) against the oracle on a query sample
