@@ -314,6 +314,13 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
         const int ntile = min(TPS, t_end - ts);
         const int buf = k % 3;
         if constexpr (HANDOFF) wait_count(&s_ready[buf], (uint32_t)(WAVES * (k / 3 + 1)), a.wait_polls, a.status);
+        // stage k+2 goes into the slot of stage k-1 once every wave has read
+        // it (hand-off: its counter; otherwise the barrier that closed stage
+        // k-1, so 4-wave blocks issue the copies before this stage's LDS reads)
+        const int b2 = (k + 2) % 3;
+        const bool issued = k + 2 < nst;
+        bool issued_full = false;
+        if constexpr (!HANDOFF) issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
         uint4 afr[TPS][G];
         int nlr[TPS][16];
         // (tiles past the slab end read stale LDS: in bounds, never flagged)
@@ -335,15 +342,11 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
             wait_lgkm0();  // this stage's fragments are in registers
             signal_count(&s_free[buf]);
         }
-        // stage k+2 goes into the slot of stage k-1 once every wave has read
-        // it (hand-off: its counter; otherwise the barrier that closed stage k-1)
-        const int b2 = (k + 2) % 3;
-        const bool issued = k + 2 < nst;
         if constexpr (HANDOFF) {
             if (issued && k >= 1)
                 wait_count(&s_free[b2], (uint32_t)(WAVES * ((k - 1) / 3 + 1)), a.wait_polls, a.status);
+            issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
         }
-        const bool issued_full = issued ? dma_stage(ts + 2 * TPS, b2) : false;
         if constexpr (HANDOFF) {
             if (k + 1 < nst) {  // this wave's copies of stage k+1 have landed
                 if (issued_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
