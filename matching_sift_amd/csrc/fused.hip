@@ -190,9 +190,15 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
     constexpr int NPW = (CH + WAVES - 1) / WAVES;
     constexpr int CHP = NPW * WAVES;
     constexpr int NLC = (TPS * 32 + 63) / 64;
-    constexpr int NPW_T = NPW + NLC;
+    // I8 brute force, 4-wave blocks: the rows' c (exact keys on the slow
+    // path) comes into LDS with the stage as well, instead of a global read
+    // per flagged tile (8-wave blocks free a slot before their slow path, so
+    // they keep the global read)
+    constexpr bool NC = KIND == KIND_I8 && !PAIRS && WAVES < 8;
+    constexpr int NPW_T = NPW + NLC * (NC ? 2 : 1);
     __shared__ __align__(16) uint4 s_tiles[3][CHP * 64];
     __shared__ __align__(16) float s_nl[3][WAVES][NLC * 64];
+    __shared__ __align__(16) int s_nc[NC ? 3 : 1][NC ? WAVES : 1][NC ? NLC * 64 : 4];
 
     const int t_begin = task.t_begin, t_end = task.t_end;
     const int tid = threadIdx.x;
@@ -267,6 +273,11 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                 for (int c = 0; c < NLC; c++)
                     dma4_s(a.b_nl + (size_t)ts * 32 + c * 64, (uint32_t)lane * 4u, lds_of(&s_nl[bsel][wave][c * 64]));
+                if constexpr (NC) {
+#pragma unroll
+                    for (int c = 0; c < NLC; c++)
+                        dma4_s(a.b_nlc + (size_t)ts * 32 + c * 64, (uint32_t)lane * 4u, lds_of(&s_nc[bsel][wave][c * 64]));
+                }
                 return true;
             }
         } else {
@@ -279,6 +290,11 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
         for (int c = 0; c < NLC; c++)
             dma4(a.b_nl + (size_t)ts * 32 + min(c * 64 + lane, nt * 32 - 1), &s_nl[bsel][wave][c * 64]);
+        if constexpr (NC) {
+#pragma unroll
+            for (int c = 0; c < NLC; c++)
+                dma4(a.b_nlc + (size_t)ts * 32 + min(c * 64 + lane, nt * 32 - 1), &s_nc[bsel][wave][c * 64]);
+        }
         return full;
     };
 
@@ -505,7 +521,8 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                             for (int j = 0; j < 16; j++) t[j] = acc[j];
                         } else {
-                            const int4* cl = reinterpret_cast<const int4*>(a.b_nlc + (size_t)(ts + tt) * 32 + h * 16);
+                            const int4* cl = NC ? reinterpret_cast<const int4*>(&s_nc[buf][wave][tt * 32 + h * 16])
+                                                : reinterpret_cast<const int4*>(a.b_nlc + (size_t)(ts + tt) * 32 + h * 16);
 #pragma unroll
                             for (int c4 = 0; c4 < 4; c4++) {
                                 const int4 cv = cl[c4];
@@ -737,15 +754,16 @@ int fused_occupancy() {
     {K, Q, W, T, fused_launch<K, Q, W, T>, fused_occupancy<K, Q, W, T>, pairs_launch<K, Q, W, T>, pairs_occupancy<K, Q, W, T>}
 
 const FusedShape kShapes[] = {
-    // default per kind comes first
-    SHAPE(KIND_BF16, 2, 8, 2),
+    // default per kind comes first (the brute force times the first two of
+    // BF16 and F16 on a large base and keeps the faster)
     SHAPE(KIND_BF16, 2, 4, 2),
+    SHAPE(KIND_BF16, 2, 8, 2),
     SHAPE(KIND_BF16, 2, 4, 4),
     SHAPE(KIND_BF16, 4, 4, 2),
     SHAPE(KIND_BF16, 1, 4, 2),
     SHAPE(KIND_BF16, 1, 8, 2),
-    SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F16, 2, 4, 2),
+    SHAPE(KIND_F16, 2, 8, 2),
     SHAPE(KIND_F32, 2, 4, 1),
     SHAPE(KIND_FP8, 4, 4, 2),
     SHAPE(KIND_FP8, 2, 4, 2),
