@@ -172,6 +172,28 @@ struct SlabTask {
 template <typename T>
 __device__ inline T min3k(T a, T b, T c) { return min(min(a, b), c); }
 
+// v_med3_i32 / v_min3_i32 written out: min/max trees over shared operands do
+// not reliably combine into them.  Plain VALU ops (not volatile), so the
+// compiler still schedules, combines and removes them.
+__device__ inline int med3_i32(int a, int b, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return max(min(a, b), min(max(a, b), c));
+#endif
+}
+__device__ inline int min3_i32(int a, int b, int c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    int r;
+    asm("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+#else
+    return min(min(a, b), c);
+#endif
+}
+
 template <int KIND, int QW, int WAVES, int TPS, bool PAIRS>
 __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& task) {
     using KTk = KT<KIND>;
@@ -428,17 +450,22 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 #pragma unroll
                 for (int qb = 0; qb < QW; qb++) {
 #pragma unroll
-                    for (int j = 0; j < 16; j++) {
-                        int cv;
-                        if constexpr (FLOAT_KEY)
-                            cv = ((int)acc[qb][j] << SB) | (NEWB + 32 * (RB ? tt : 0) + 8 * (j >> 2) + 4 * h + (j & 3));
-                        else
-                            cv = __mul24((int)acc[qb][j], a.comp_mul) + nlr[tt][j];
-                        // c1 <= c2 always, so the new 2nd smallest of the three
-                        // is their median
-                        const int lo = min(c1[qb], c2[qb]), hi = max(c1[qb], c2[qb]);
-                        c2[qb] = max(lo, min(hi, cv));  // v_med3_i32
-                        c1[qb] = min(c1[qb], cv);
+                    for (int j = 0; j < 16; j += 2) {
+                        int cv[2];
+#pragma unroll
+                        for (int e = 0; e < 2; e++) {
+                            const int jj = j + e;
+                            if constexpr (FLOAT_KEY)
+                                cv[e] = ((int)acc[qb][jj] << SB) |
+                                        (NEWB + 32 * (RB ? tt : 0) + 8 * (jj >> 2) + 4 * h + (jj & 3));
+                            else
+                                cv[e] = __mul24((int)acc[qb][jj], a.comp_mul) + nlr[tt][jj];
+                        }
+                        // two rows at once (c1 <= c2): the 2nd smallest of
+                        // {c1, x, y} is their median, and c2 only enters if it is
+                        // below that; the smallest is min3.  3 VALU per two rows.
+                        c2[qb] = min(c2[qb], med3_i32(c1[qb], cv[0], cv[1]));
+                        c1[qb] = min3_i32(c1[qb], cv[0], cv[1]);
                     }
                 }
                 // after the stage's last tile (or the slab's): new entries
