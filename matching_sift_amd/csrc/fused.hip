@@ -342,6 +342,18 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
 
     const int nst = (t_end - t_begin + TPS - 1) / TPS;  // stages of this slab
     if (nst > 0) dma_stage(t_begin, 0);
+    // The query fragments are complete from here on, as far as the compiler
+    // can tell: it does not see the stage copies (inline asm), so without
+    // this it waits for the fragment loads at their first use inside the
+    // stage loop, with vmcnt values that the copies in flight then make wait
+    // for the copies issued two stages ahead.  The wait it places here
+    // covers the fragments and stage 0's copies, which the first stage needs
+    // anyway; stage 1's copies are issued after it.
+#pragma unroll
+    for (int qb = 0; qb < QW; qb++)
+#pragma unroll
+        for (int g = 0; g < G; g++)
+            asm volatile("" : "+v"(qop[qb][g].x), "+v"(qop[qb][g].y), "+v"(qop[qb][g].z), "+v"(qop[qb][g].w));
     bool next_full = nst > 1 ? dma_stage(t_begin + TPS, 1) : false;
     if (nst > 0) {
         if (next_full) wait_vm_lgkm0<NPW_T>(); else wait_vm_lgkm0<0>();
