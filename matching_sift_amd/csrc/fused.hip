@@ -184,6 +184,17 @@ __device__ inline int med3_i32(int a, int b, int c) {
     return max(min(a, b), min(max(a, b), c));
 #endif
 }
+// (m & a) | (~m & b) in one v_bfi_b32 (the compiler turns the expression back
+// into a compare and a select when m is a sign-extended bit).
+__device__ inline uint32_t bfi_b32(uint32_t m, uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
+#else
+    return (m & a) | (~m & b);
+#endif
+}
 __device__ inline int min3_i32(int a, int b, int c) {
 #if defined(__HIP_DEVICE_COMPILE__)
     int r;
@@ -486,12 +497,18 @@ __device__ __forceinline__ void fused_slab(const FusedArgs& a, const SlabTask& t
                     const uint32_t row0 = (uint32_t)(ts + (RB ? 0 : tt)) * 32u;
 #pragma unroll
                     for (int qb = 0; qb < QW; qb++) {
-                        const bool n1 = (c1[qb] & NEWB) != 0, n2 = (c2[qb] & NEWB) != 0;
-                        const uint32_t r1 = row0 + (uint32_t)(c1[qb] & (NEWB - 1));
-                        const uint32_t r2 = row0 + (uint32_t)(c2[qb] & (NEWB - 1));
+                        // all ones where the entry is new (v_bfe_i32), and the
+                        // selects as bit-field inserts (v_bfi_b32)
+                        const uint32_t n1 = (uint32_t)((c1[qb] << (32 - SB)) >> 31);
+                        const uint32_t n2 = (uint32_t)((c2[qb] << (32 - SB)) >> 31);
+                        const uint32_t s1 = (uint32_t)(c1[qb] & (NEWB - 1)), s2 = (uint32_t)(c2[qb] & (NEWB - 1));
+                        // stage rows start on multiples of NEWB except in FP8's
+                        // image quarters: there the slot is added, else OR-ed in
+                        const uint32_t r1 = KIND == KIND_FP8 ? row0 + s1 : (row0 | s1);
+                        const uint32_t r2 = KIND == KIND_FP8 ? row0 + s2 : (row0 | s2);
                         // a kept 2nd entry is the old 1st if the 1st was displaced
-                        i2[qb] = n2 ? r2 : (n1 ? i1[qb] : i2[qb]);
-                        i1[qb] = n1 ? r1 : i1[qb];
+                        i2[qb] = bfi_b32(n2, r2, bfi_b32(n1, i1[qb], i2[qb]));
+                        i1[qb] = bfi_b32(n1, r1, i1[qb]);
                         c1[qb] &= ~((1 << SB) - 1);
                         c2[qb] &= ~((1 << SB) - 1);
                     }
